@@ -1,0 +1,171 @@
+"""Throughput of the MI355X statevector engine on the 30-qubit workloads.
+
+One step = one pass of the hot path over one batch: the whole 30-qubit
+U3+CX brickwork circuit of depth 40 (1780 gates: 1200 U3 + 580 CX, angles
+from a fixed seed) applied to a complex128 state already resident in HBM.
+Prints one JSON line: gates/s (value), achieved HBM GB/s, the roofline of the
+tile-pass kernel (HIP events on the engine's stream), the complex64 variant,
+the 30-qubit QFT, and a CPU baseline (the oracle restatement of the
+reference's OpenMP CPU path on a bounded sample of the same workload).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--qubits 30]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import pathlib
+import sys
+import time
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_engine(n: int, precision: str, circ, steps: int, warmup: int, init=None) -> dict:
+    from qiskit_aer_amd import QubitVector
+    qv = QubitVector(n, precision)
+    if init is None:
+        qv.initialize()
+    else:
+        init(qv)
+    packed = circ.pack()
+    for _ in range(warmup):
+        qv.apply_ops(packed)
+        qv.synchronize()
+    st0 = qv.stats()
+    qv.set_event_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        qv.apply_ops(packed)
+        qv.synchronize()
+    t1 = time.perf_counter()
+    ms, launches = qv.pass_timing()
+    st1 = qv.stats()
+    norm = qv.norm()
+    qv.close()
+    amp = 16 if precision == "double" else 8
+    sweep_bytes = 2 * (1 << n) * amp
+    passes = st1["passes"] - st0["passes"]
+    wall = t1 - t0
+    gates = circ.count_gates() * steps
+    return {
+        "wall_s": wall,
+        "ms_per_step": 1e3 * wall / steps,
+        "gates_per_s": gates / wall,
+        "passes_per_step": passes / steps,
+        "hbm_gbs": passes * sweep_bytes / wall / 1e9,
+        "kernel_ms_avg": ms / max(launches, 1),
+        "kernel_gbs": sweep_bytes / (ms / max(launches, 1) / 1e3) / 1e9 if launches else None,
+        "sweep_bytes": sweep_bytes,
+        "norm_after": norm,
+    }
+
+
+def cpu_baseline(n: int, layers: int, seed: int) -> dict:
+    """The oracle (restated reference CPU path, fusion on, OpenMP on every
+    host core) on the first `layers` layers of the same brickwork."""
+    from oracle import pyoracle as O
+    from qiskit_aer_amd import circuits
+    c = circuits.brickwork_u3_cx(n, layers, seed, measure=False)
+    threads = len(os.sched_getaffinity(0))
+    _, _, timing = O.run_circuit(c, shots=0, seed=0, threads=threads, statevector=False)
+    return {"value": c.count_gates() / timing[0], "unit": "gates/s", "cores": threads, "kind": "port",
+            "sample": f"{n}q c128 brickwork, first {layers} of 40 layers ({c.count_gates()} gates), "
+                      f"Aer fusion on (max 5), gate phase {timing[0]:.2f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--qubits", type=int, default=30)
+    ap.add_argument("--layers", type=int, default=40)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+              "the statevector is not sharded across devices.", file=sys.stderr)
+        return 2
+
+    from qiskit_aer_amd import circuits
+    n, seed = args.qubits, 2024
+    circ = circuits.brickwork_u3_cx(n, args.layers, seed, measure=False)
+    log(f"bench: brickwork {n}q depth {args.layers} c128, {circ.count_gates()} gates")
+    c128 = run_engine(n, "double", circ, args.steps, args.warmup)
+    log(f"bench: c128 {c128['gates_per_s']:.1f} gates/s, {c128['hbm_gbs']:.0f} GB/s, "
+        f"kernel {c128['kernel_gbs']:.0f} GB/s")
+    extra = {}
+    if not args.no_extra:
+        c64 = run_engine(n, "single", circ, args.steps, args.warmup)
+        log(f"bench: c64 {c64['gates_per_s']:.1f} gates/s, {c64['hbm_gbs']:.0f} GB/s")
+        extra["c64"] = {k: c64[k] for k in ("gates_per_s", "ms_per_step", "hbm_gbs", "kernel_gbs",
+                                             "passes_per_step")}
+        extra["c64_over_c128"] = c64["gates_per_s"] / c128["gates_per_s"]
+        q = circuits.qft(n)
+
+        def gaussian(qv):
+            try:
+                import torch
+                g = torch.randn(1 << n, dtype=torch.complex128, device="cuda")
+                g /= torch.linalg.vector_norm(g)
+                torch.cuda.synchronize()
+                qv.initialize_from_device(g.data_ptr(), 1 << n, "double")
+                del g
+                torch.cuda.empty_cache()
+            except Exception as e:  # torch missing: a basis state still measures throughput
+                log(f"bench: QFT input from |0> ({e})")
+                qv.initialize()
+        qres = run_engine(n, "double", q, max(1, args.steps), 1, init=gaussian)
+        log(f"bench: qft {qres['gates_per_s']:.1f} gates/s")
+        extra["qft_c128"] = {k: qres[k] for k in ("gates_per_s", "ms_per_step", "hbm_gbs",
+                                                   "passes_per_step")}
+        extra["qft_c128"]["gates"] = q.count_gates()
+
+    cpu = None
+    if not args.no_cpu:
+        log("bench: cpu baseline")
+        cpu = cpu_baseline(n, 2, seed)
+        log(f"bench: cpu {cpu['value']:.2f} gates/s on {cpu['cores']} cores")
+
+    line = {
+        "metric": "gates/s + HBM GB/s at 30q (c64 vs c128); adjoint grads/s vs param-shift, 24q",
+        "value": c128["gates_per_s"],
+        "unit": "gates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": c128["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "complex128",
+        "data": "synthetic (seeded random U3 angles)",
+        "config": {"workload": f"brickwork_u3_cx_{n}q_depth{args.layers}", "num_qubits": n,
+                   "layers": args.layers, "gates": circ.count_gates(), "precision": "c128",
+                   "parallelism": "single GPU"},
+        "hbm_gbs": c128["hbm_gbs"],
+        "passes_per_step": c128["passes_per_step"],
+        "roofline": {"bound": "hbm", "achieved": c128["kernel_gbs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": c128["kernel_gbs"] / HBM_PEAK_GBS,
+                     "traffic": None},
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,432 @@
+// Controller slice: runs one circuit on the HIP engine through the C ABI.
+//
+// It restates what sits above the QubitVector seam in the reference for the
+// statevector method:
+//   - gate name -> engine call (Statevector::State::apply_gate,
+//     src/simulators/statevector/statevector_state.hpp:733-865);
+//   - matrix / diagonal ops (statevector_state.hpp:879-915);
+//   - measure / reset / initialize (statevector_state.hpp:936-1085);
+//   - the sampling driver (Executor::run_circuit_with_sampling and
+//     measure_sampler, src/simulators/circuit_executor.hpp:736-825,
+//     1031-1161) with mt19937_64 draws (src/framework/rng.hpp:32-101).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <map>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "aerhip.h"
+#include "gate_matrices.hpp"
+
+// Sets the thread-local message returned by aerhip_last_error (engine.hip).
+extern "C" __attribute__((visibility("hidden"))) void aerhip_detail_set_error(const char *msg);
+
+namespace {
+
+using cd = std::complex<double>;
+namespace G = aerhip::gates;
+
+
+enum class Gate {
+  id, x, y, mcz, s, sdg, h, t, tdg, mcp, mcsx, mcsxdg, mcr, mcrx, mcry, mcrz,
+  mcu2, mcu3, mcu, mcswap, rxx, ryy, rzz, rzx, ecr, pauli
+};
+
+// The statevector gate set (statevector_state.hpp:314-384); "x"-family and
+// "y"-family names map to mcx / mcy.
+const std::unordered_map<std::string, Gate> &gate_set() {
+  static const std::unordered_map<std::string, Gate> m = {
+      {"delay", Gate::id}, {"id", Gate::id}, {"x", Gate::x}, {"y", Gate::y},
+      {"z", Gate::mcz}, {"s", Gate::s}, {"sdg", Gate::sdg}, {"h", Gate::h},
+      {"t", Gate::t}, {"tdg", Gate::tdg}, {"p", Gate::mcp}, {"sx", Gate::mcsx},
+      {"sxdg", Gate::mcsxdg}, {"r", Gate::mcr}, {"rx", Gate::mcrx}, {"ry", Gate::mcry},
+      {"rz", Gate::mcrz}, {"u1", Gate::mcp}, {"u2", Gate::mcu2}, {"u3", Gate::mcu3},
+      {"u", Gate::mcu3}, {"U", Gate::mcu3}, {"CX", Gate::x}, {"cx", Gate::x},
+      {"cy", Gate::y}, {"cz", Gate::mcz}, {"cp", Gate::mcp}, {"cu1", Gate::mcp},
+      {"cu2", Gate::mcu2}, {"cu3", Gate::mcu3}, {"cu", Gate::mcu}, {"swap", Gate::mcswap},
+      {"rxx", Gate::rxx}, {"ryy", Gate::ryy}, {"rzz", Gate::rzz}, {"rzx", Gate::rzx},
+      {"csx", Gate::mcsx}, {"csxdg", Gate::mcsxdg}, {"ecr", Gate::ecr}, {"crx", Gate::mcrx},
+      {"cry", Gate::mcry}, {"crz", Gate::mcrz}, {"ccx", Gate::x}, {"ccz", Gate::mcz},
+      {"cswap", Gate::mcswap}, {"mcx", Gate::x}, {"mcy", Gate::y}, {"mcz", Gate::mcz},
+      {"mcr", Gate::mcr}, {"mcrx", Gate::mcrx}, {"mcry", Gate::mcry}, {"mcrz", Gate::mcrz},
+      {"mcphase", Gate::mcp}, {"mcp", Gate::mcp}, {"mcu1", Gate::mcp}, {"mcu2", Gate::mcu2},
+      {"mcu3", Gate::mcu3}, {"mcu", Gate::mcu}, {"mcswap", Gate::mcswap}, {"mcsx", Gate::mcsx},
+      {"mcsxdg", Gate::mcsxdg}, {"pauli", Gate::pauli}, {"mcx_gray", Gate::x}};
+  return m;
+}
+
+void ck(int rc) {
+  if (rc != 0) throw std::runtime_error(aerhip_last_error());
+}
+
+struct Runner {
+  aerhip_qv *qv = nullptr;
+  uint64_t n = 0;
+  bool owns = true;
+  std::mt19937_64 rng;
+
+  ~Runner() {
+    if (qv && owns) aerhip_qv_destroy(qv);
+  }
+
+  double rand01() { return std::uniform_real_distribution<double>(0, 1)(rng); }
+
+  void mcu(const std::vector<uint64_t> &q, const G::cvec &m) {
+    ck(aerhip_apply_mcu(qv, q.data(), q.size(), reinterpret_cast<const double *>(m.data())));
+  }
+  void diag(const std::vector<uint64_t> &q, const G::cvec &d) {
+    ck(aerhip_apply_diagonal_matrix(qv, q.data(), q.size(),
+                                    reinterpret_cast<const double *>(d.data())));
+  }
+  void matrix(const std::vector<uint64_t> &q, const G::cvec &m) {
+    ck(aerhip_apply_matrix(qv, q.data(), q.size(), reinterpret_cast<const double *>(m.data())));
+  }
+
+  void apply_gate(const aerhip_op &op, const std::vector<uint64_t> &q) {
+    const std::string name = op.name ? op.name : "";
+    auto it = gate_set().find(name);
+    if (it == gate_set().end())
+      throw std::invalid_argument("QubitVectorState::invalid gate instruction '" + name + "'.");
+    auto P = [&](uint32_t i) -> double {
+      if (i >= op.num_params)
+        throw std::invalid_argument("gate '" + name + "' is missing parameter " +
+                                    std::to_string(i));
+      return op.params[i];
+    };
+    const double isqrt2 = 1. / std::sqrt(2);
+    switch (it->second) {
+      case Gate::x: ck(aerhip_apply_mcx(qv, q.data(), q.size())); break;
+      case Gate::y: ck(aerhip_apply_mcy(qv, q.data(), q.size())); break;
+      case Gate::mcz: ck(aerhip_apply_mcphase(qv, q.data(), q.size(), -1.0, 0.0)); break;
+      case Gate::mcr: mcu(q, G::r(P(0), P(1))); break;
+      case Gate::mcrx: ck(aerhip_apply_rotation(qv, q.data(), q.size(), AERHIP_ROT_X, P(0))); break;
+      case Gate::mcry: ck(aerhip_apply_rotation(qv, q.data(), q.size(), AERHIP_ROT_Y, P(0))); break;
+      case Gate::mcrz: ck(aerhip_apply_rotation(qv, q.data(), q.size(), AERHIP_ROT_Z, P(0))); break;
+      case Gate::rxx: ck(aerhip_apply_rotation(qv, q.data(), q.size(), AERHIP_ROT_XX, P(0))); break;
+      case Gate::ryy: ck(aerhip_apply_rotation(qv, q.data(), q.size(), AERHIP_ROT_YY, P(0))); break;
+      case Gate::rzz: ck(aerhip_apply_rotation(qv, q.data(), q.size(), AERHIP_ROT_ZZ, P(0))); break;
+      case Gate::rzx: ck(aerhip_apply_rotation(qv, q.data(), q.size(), AERHIP_ROT_ZX, P(0))); break;
+      case Gate::ecr: matrix(q, G::ecr()); break;
+      case Gate::id: break;
+      case Gate::h: mcu(q, G::u4(M_PI / 2., 0., M_PI, 0.)); break;
+      case Gate::s: diag({q[0]}, {1., cd(0., 1.)}); break;
+      case Gate::sdg: diag({q[0]}, {1., cd(0., -1.)}); break;
+      case Gate::t: diag({q[0]}, {1., cd(isqrt2, isqrt2)}); break;
+      case Gate::tdg: diag({q[0]}, {1., cd(isqrt2, -isqrt2)}); break;
+      case Gate::mcswap: ck(aerhip_apply_mcswap(qv, q.data(), q.size())); break;
+      case Gate::mcu3: mcu(q, G::u4(P(0), P(1), P(2), 0.)); break;
+      case Gate::mcu: mcu(q, G::u4(P(0), P(1), P(2), P(3))); break;
+      case Gate::mcu2: mcu(q, G::u4(M_PI / 2., P(0), P(1), 0.)); break;
+      case Gate::mcp: {
+        const cd ph = std::exp(cd(0, 1) * cd(P(0), 0.));
+        ck(aerhip_apply_mcphase(qv, q.data(), q.size(), ph.real(), ph.imag()));
+        break;
+      }
+      case Gate::mcsx: mcu(q, G::sx()); break;
+      case Gate::mcsxdg: mcu(q, G::sxdg()); break;
+      case Gate::pauli: {
+        if (!op.string_param) throw std::invalid_argument("pauli gate needs a label");
+        ck(aerhip_apply_pauli(qv, q.data(), q.size(), op.string_param, 1.0, 0.0));
+        break;
+      }
+    }
+  }
+
+  // State::apply_matrix (statevector_state.hpp:879-899): exactly diagonal
+  // matrices take the diagonal path.
+  void apply_matrix_op(const aerhip_op &op, const std::vector<uint64_t> &q) {
+    const uint64_t dim = 1ull << q.size();
+    if (op.num_params != dim * dim) throw std::invalid_argument("matrix op has the wrong size");
+    const cd *m = reinterpret_cast<const cd *>(op.params);
+    bool diagonal = true;
+    for (uint64_t j = 0; j < dim && diagonal; ++j)
+      for (uint64_t i = 0; i < dim; ++i)
+        if (i != j && std::abs(m[i + dim * j]) > 0.0) {
+          diagonal = false;
+          break;
+        }
+    if (diagonal) {
+      G::cvec d(dim);
+      for (uint64_t i = 0; i < dim; ++i) d[i] = m[i + dim * i];
+      diag(q, d);
+    } else {
+      ck(aerhip_apply_matrix(qv, q.data(), q.size(), op.params));
+    }
+  }
+
+  // sample_measure_with_prob + measure_reset_update
+  // (statevector_state.hpp:950-1014).
+  uint64_t measure_collapse(const std::vector<uint64_t> &q, bool reset) {
+    const uint64_t dim = 1ull << q.size();
+    std::vector<double> probs(dim);
+    ck(aerhip_probabilities(qv, q.data(), q.size(), probs.data()));
+    const uint64_t meas = std::discrete_distribution<size_t>(probs.begin(), probs.end())(rng);
+    const double p = probs[meas];
+    const uint64_t final_state = reset ? 0 : meas;
+    G::cvec mdiag(dim, 0.);
+    mdiag[meas] = 1. / std::sqrt(p);
+    diag(q, mdiag);
+    if (final_state != meas) {
+      if (q.size() == 1) {
+        ck(aerhip_apply_mcx(qv, q.data(), q.size()));
+      } else {
+        G::cvec perm(dim * dim, 0.);
+        perm[final_state * dim + meas] = 1.;
+        perm[meas * dim + final_state] = 1.;
+        for (uint64_t j = 0; j < dim; ++j)
+          if (j != final_state && j != meas) perm[j * dim + j] = 1.;
+        matrix(q, perm);
+      }
+    }
+    return meas;
+  }
+};
+
+bool is_measure_tail(const aerhip_op *ops, uint64_t num_ops, uint64_t first) {
+  for (uint64_t i = first; i < num_ops; ++i)
+    if (ops[i].type != AERHIP_OP_MEASURE && ops[i].type != AERHIP_OP_BARRIER) return false;
+  return true;
+}
+
+std::vector<uint64_t> op_qubits(const aerhip_op &op, uint64_t n) {
+  std::vector<uint64_t> q(op.qubits, op.qubits + op.num_qubits);
+  for (uint64_t v : q)
+    if (v >= n) throw std::invalid_argument("operation qubit out of range");
+  return q;
+}
+
+}  // namespace
+
+extern "C" {
+
+void aerhip_run_config_default(aerhip_run_config *cfg) {
+  if (!cfg) return;
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->precision = AERHIP_PRECISION_DOUBLE;
+  cfg->device = 0;
+  cfg->shots = 1024;
+  cfg->seed = 0;
+  cfg->fusion_enable = 1;
+  cfg->fusion_max_qubit = 5;
+  cfg->fusion_threshold = 14;
+  cfg->sample_index_size = 10;
+  cfg->global_phase = 0.0;
+}
+
+int aerhip_run_circuit(const aerhip_op *ops, uint64_t num_ops, uint64_t num_qubits,
+                       uint64_t num_memory, const aerhip_run_config *cfg_in,
+                       uint64_t *memory_out, double *statevector_out, double *timing_out) {
+  try {
+    if (num_ops && !ops) throw std::invalid_argument("null op list");
+    if (num_memory > 64) throw std::invalid_argument("at most 64 classical bits are supported");
+    aerhip_run_config cfg;
+    if (cfg_in) cfg = *cfg_in; else aerhip_run_config_default(&cfg);
+    Runner R;
+    R.n = num_qubits;
+    R.qv = aerhip_qv_create(num_qubits, cfg.precision, cfg.device);
+    if (!R.qv) throw std::runtime_error(aerhip_last_error());
+    ck(aerhip_qv_set_sample_measure_index_size(R.qv, cfg.sample_index_size));
+
+    uint64_t first_meas = num_ops;
+    for (uint64_t i = 0; i < num_ops; ++i)
+      if (ops[i].type == AERHIP_OP_MEASURE) {
+        first_meas = i;
+        break;
+      }
+    const bool can_sample = is_measure_tail(ops, num_ops, first_meas);
+
+    // Global phase (State::apply_global_phase, statevector_state.hpp:446-450);
+    // angles within 1e-10 of zero are treated as none (state.hpp:398-414).
+    const bool has_gphase = std::abs(cfg.global_phase) > 1e-10;
+    const cd gphase = std::exp(cd(0.0, cfg.global_phase));
+
+    bool saved = false;
+    double t_gates = 0, t_sample = 0;
+    using clk = std::chrono::steady_clock;
+
+    auto run_ops = [&](uint64_t begin, uint64_t end, std::vector<uint64_t> *creg) {
+      for (uint64_t i = begin; i < end; ++i) {
+        const aerhip_op &op = ops[i];
+        const std::vector<uint64_t> q = op_qubits(op, num_qubits);
+        switch (op.type) {
+          case AERHIP_OP_GATE: R.apply_gate(op, q); break;
+          case AERHIP_OP_MATRIX: R.apply_matrix_op(op, q); break;
+          case AERHIP_OP_DIAGONAL: {
+            if (op.num_params != (1ull << q.size()))
+              throw std::invalid_argument("diagonal op has the wrong size");
+            ck(aerhip_apply_diagonal_matrix(R.qv, q.data(), q.size(), op.params));
+            break;
+          }
+          case AERHIP_OP_BARRIER: break;
+          case AERHIP_OP_MEASURE: {
+            const uint64_t m = R.measure_collapse(q, false);
+            if (creg && op.memory)
+              for (size_t j = 0; j < q.size(); ++j) {
+                const uint64_t b = (m >> j) & 1ull;
+                (*creg)[0] = ((*creg)[0] & ~(1ull << op.memory[j])) | (b << op.memory[j]);
+              }
+            break;
+          }
+          case AERHIP_OP_RESET: R.measure_collapse(q, true); break;
+          case AERHIP_OP_INITIALIZE: {
+            const uint64_t dim = 1ull << q.size();
+            if (op.num_params != dim) throw std::invalid_argument("initialize has the wrong size");
+            std::vector<cd> v(reinterpret_cast<const cd *>(op.params),
+                              reinterpret_cast<const cd *>(op.params) + dim);
+            if (has_gphase)
+              for (auto &x : v) x *= gphase;
+            bool full = q.size() == num_qubits;
+            for (size_t j = 0; full && j < q.size(); ++j) full = q[j] == j;
+            if (full) {
+              ck(aerhip_qv_initialize_from_vector(R.qv, reinterpret_cast<const double *>(v.data()),
+                                                  dim));
+            } else {
+              R.measure_collapse(q, true);
+              ck(aerhip_qv_initialize_component(R.qv, q.data(), q.size(),
+                                                reinterpret_cast<const double *>(v.data())));
+            }
+            break;
+          }
+          case AERHIP_OP_SAVE_STATEVECTOR:
+            if (statevector_out && !saved) {
+              ck(aerhip_qv_copy_to_vector(R.qv, statevector_out, 1ull << num_qubits));
+              saved = true;
+            }
+            break;
+          default: throw std::invalid_argument("unknown operation type");
+        }
+      }
+    };
+    auto init_state = [&] {
+      ck(aerhip_qv_initialize(R.qv));
+      if (has_gphase) {
+        const uint64_t q0 = 0;
+        const cd d[2] = {gphase, gphase};
+        ck(aerhip_apply_diagonal_matrix(R.qv, &q0, 1, reinterpret_cast<const double *>(d)));
+      }
+    };
+
+    if (can_sample) {
+      R.rng.seed(cfg.seed);
+      const auto t0 = clk::now();
+      init_state();
+      run_ops(0, first_meas, nullptr);
+      ck(aerhip_qv_synchronize(R.qv));
+      t_gates = std::chrono::duration<double>(clk::now() - t0).count();
+      if (statevector_out && !saved) {
+        ck(aerhip_qv_copy_to_vector(R.qv, statevector_out, 1ull << num_qubits));
+        saved = true;
+      }
+      // measure_sampler: measured qubits sorted and unique.
+      std::vector<uint64_t> mq;
+      for (uint64_t i = first_meas; i < num_ops; ++i)
+        if (ops[i].type == AERHIP_OP_MEASURE)
+          for (uint32_t j = 0; j < ops[i].num_qubits; ++j) mq.push_back(ops[i].qubits[j]);
+      std::sort(mq.begin(), mq.end());
+      mq.erase(std::unique(mq.begin(), mq.end()), mq.end());
+      if (!mq.empty() && cfg.shots > 0) {
+        const auto t1 = clk::now();
+        std::vector<double> rnds(cfg.shots);
+        for (auto &r : rnds) r = R.rand01();
+        std::vector<uint64_t> samples(cfg.shots);
+        ck(aerhip_sample_measure(R.qv, rnds.data(), rnds.size(), samples.data()));
+        t_sample = std::chrono::duration<double>(clk::now() - t1).count();
+        // memory bit -> position of its qubit among the measured qubits
+        std::map<uint64_t, uint64_t> memory_map;
+        for (uint64_t i = first_meas; i < num_ops; ++i)
+          if (ops[i].type == AERHIP_OP_MEASURE && ops[i].memory)
+            for (uint32_t j = 0; j < ops[i].num_qubits; ++j) {
+              const uint64_t pos = std::lower_bound(mq.begin(), mq.end(), ops[i].qubits[j]) - mq.begin();
+              memory_map[ops[i].memory[j]] = pos;
+            }
+        if (memory_out) {
+          for (uint64_t is = 0; is < cfg.shots; ++is) {
+            const uint64_t s = samples[cfg.shots - is - 1];  // reference memory order
+            uint64_t mem = 0;
+            for (const auto &kv : memory_map) {
+              const uint64_t qubit = mq[kv.second];
+              mem |= ((s >> qubit) & 1ull) << kv.first;
+            }
+            memory_out[is] = mem;
+          }
+        }
+      } else if (memory_out) {
+        for (uint64_t is = 0; is < cfg.shots; ++is) memory_out[is] = 0;
+      }
+    } else {
+      // Shot by shot (State::apply_measure with per-shot seeds,
+      // circuit_executor.hpp:906-908).
+      const auto t0 = clk::now();
+      for (uint64_t s = 0; s < std::max<uint64_t>(cfg.shots, 1); ++s) {
+        R.rng.seed(cfg.seed + s);
+        init_state();
+        std::vector<uint64_t> creg(1, 0);
+        run_ops(0, num_ops, &creg);
+        if (memory_out && s < cfg.shots) memory_out[s] = creg[0];
+      }
+      ck(aerhip_qv_synchronize(R.qv));
+      t_gates = std::chrono::duration<double>(clk::now() - t0).count();
+    }
+    if (timing_out) {
+      uint64_t st[3] = {0, 0, 0};
+      ck(aerhip_qv_stats(R.qv, st));
+      timing_out[0] = t_gates;
+      timing_out[1] = t_sample;
+      timing_out[2] = static_cast<double>(st[0]);
+      timing_out[3] = static_cast<double>(st[1]);
+    }
+    return 0;
+  } catch (const std::invalid_argument &e) {
+    aerhip_detail_set_error(e.what());
+    return 2;
+  } catch (const std::exception &e) {
+    aerhip_detail_set_error(e.what());
+    return 1;
+  }
+}
+
+// State::apply_ops (src/simulators/state.hpp:295-340) for the unitary
+// operations of the wire format: applies gates, matrices and diagonals to an
+// existing statevector handle (they are queued on the engine).
+int aerhip_apply_ops(aerhip_qv *qv, const aerhip_op *ops, uint64_t num_ops) {
+  try {
+    if (!qv) throw std::invalid_argument("aerhip: null statevector handle");
+    if (num_ops && !ops) throw std::invalid_argument("null op list");
+    Runner R;
+    R.qv = qv;
+    R.owns = false;
+    ck(aerhip_qv_num_qubits(qv, &R.n));
+    for (uint64_t i = 0; i < num_ops; ++i) {
+      const aerhip_op &op = ops[i];
+      const std::vector<uint64_t> q = op_qubits(op, R.n);
+      switch (op.type) {
+        case AERHIP_OP_GATE: R.apply_gate(op, q); break;
+        case AERHIP_OP_MATRIX: R.apply_matrix_op(op, q); break;
+        case AERHIP_OP_DIAGONAL:
+          if (op.num_params != (1ull << q.size()))
+            throw std::invalid_argument("diagonal op has the wrong size");
+          ck(aerhip_apply_diagonal_matrix(qv, q.data(), q.size(), op.params));
+          break;
+        case AERHIP_OP_BARRIER: break;
+        default:
+          throw std::invalid_argument("aerhip_apply_ops: only gate, matrix, diagonal and barrier "
+                                      "operations can be applied to a handle");
+      }
+    }
+    return 0;
+  } catch (const std::invalid_argument &e) {
+    aerhip_detail_set_error(e.what());
+    return 2;
+  } catch (const std::exception &e) {
+    aerhip_detail_set_error(e.what());
+    return 1;
+  }
+}
+
+}  // extern "C"

