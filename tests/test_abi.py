@@ -53,3 +53,22 @@ def test_no_device_means_error_not_fallback():
     from qiskit_aer_amd import QubitVector
     with pytest.raises(L.AerhipError):
         QubitVector(4)
+
+
+def test_cpp_adaptor_compiles_and_links(tmp_path):
+    """include/qubitvector_hip.hpp (the QubitVector drop-in) builds against the ABI."""
+    import shutil
+    import subprocess
+    if not shutil.which("g++"):
+        pytest.skip("g++ not available")
+    root = HEADER.parent.parent
+    src = tmp_path / "a.cpp"
+    src.write_text('#include "qubitvector_hip.hpp"\n'
+                   "template class aerhip::QubitVectorHip<double>;\n"
+                   "template class aerhip::QubitVectorHip<float>;\n"
+                   "int main() { return 0; }\n")
+    out = tmp_path / "a.out"
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", f"-I{root / 'include'}", str(src),
+                        f"-L{L.LIB_PATH.parent}", "-laerhip", f"-Wl,-rpath,{L.LIB_PATH.parent}",
+                        "-o", str(out)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
