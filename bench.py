@@ -63,12 +63,68 @@ def run_engine(n: int, precision: str, circ, steps: int, warmup: int, init=None)
         "ms_per_step": 1e3 * wall / steps,
         "gates_per_s": gates / wall,
         "passes_per_step": passes / steps,
+        "gathers_per_step": (st1["gathers"] - st0["gathers"]) / steps,
         "hbm_gbs": passes * sweep_bytes / wall / 1e9,
         "kernel_ms_avg": ms / max(launches, 1),
         "kernel_gbs": sweep_bytes / (ms / max(launches, 1) / 1e3) / 1e9 if launches else None,
         "sweep_bytes": sweep_bytes,
         "norm_after": norm,
     }
+
+
+def tfim(n: int, h: float = 0.7) -> list:
+    """-sum Z_i Z_{i+1} - h sum X_i as (label, coefficient) pairs; label
+    character n-1-i acts on qubit i."""
+    terms = []
+    for i in range(n - 1):
+        lab = ["I"] * n
+        lab[n - 1 - i] = lab[n - 2 - i] = "Z"
+        terms.append(("".join(lab), -1.0))
+    for i in range(n):
+        lab = ["I"] * n
+        lab[n - 1 - i] = "X"
+        terms.append(("".join(lab), -h))
+    return terms
+
+
+def ry_rz_cz_ansatz(n: int, layers: int):
+    """RY and RZ on every qubit then a CZ brickwork, per layer: the circuit,
+    the parameter index of each op (-1 for CZ) and the parameter count."""
+    from qiskit_aer_amd.circuit import Circuit
+    c = Circuit(n)
+    idx, p = [], 0
+    for layer in range(layers):
+        for q in range(n):
+            c.gate("ry", [q], [0.0]); idx.append(p); p += 1
+            c.gate("rz", [q], [0.0]); idx.append(p); p += 1
+        for a in range(layer % 2, n - 1, 2):
+            c.gate("cz", [a, a + 1]); idx.append(-1)
+    return c, idx, p
+
+
+def run_adjoint(n: int, layers: int, batch: int, shift_params: int, seed: int) -> dict:
+    """Energy + full gradient for `batch` parameter rows by the adjoint sweep,
+    against the parameter-shift rule on the same engine for `shift_params`
+    parameters of one row (two energy evaluations each)."""
+    from qiskit_aer_amd.adjoint import AdjointEstimator
+    c, idx, npar = ry_rz_cz_ansatz(n, layers)
+    est = AdjointEstimator(c, idx, npar, tfim(n))
+    rows = np.random.default_rng(seed).uniform(-np.pi, np.pi, size=(batch, npar))
+    est.energy_and_gradient(rows[:1])  # warm-up
+    t0 = time.perf_counter()
+    energy, grad = est.energy_and_gradient(rows)
+    t_adj = time.perf_counter() - t0
+    which = list(range(0, npar, max(1, npar // shift_params)))[:shift_params]
+    t0 = time.perf_counter()
+    ps = est.parameter_shift(rows[:1], which=which)
+    t_ps = time.perf_counter() - t0
+    adj_rate = batch * npar / t_adj
+    ps_rate = len(which) / t_ps
+    return {"grads_per_s": adj_rate, "param_shift_grads_per_s": ps_rate,
+            "speedup_vs_param_shift": adj_rate / ps_rate, "num_params": npar,
+            "batch": batch, "gates": c.count_gates(), "terms": 2 * n - 1,
+            "adjoint_s": t_adj, "param_shift_sample": f"{len(which)} params of row 0, {t_ps:.2f} s",
+            "max_abs_adjoint_minus_shift": float(np.max(np.abs(grad[0, which] - ps[0])))}
 
 
 def cpu_baseline(n: int, layers: int, seed: int) -> dict:
@@ -84,7 +140,40 @@ def cpu_baseline(n: int, layers: int, seed: int) -> dict:
                       f"Aer fusion on (max 5), gate phase {timing[0]:.2f} s"}
 
 
+def profile_summary(trace_db: str, fetch_db: str = None, write_db: str = None) -> str:
+    """Text summary of rocprofv3 output databases: per-kernel calls and average
+    duration (kernel trace), and FETCH_SIZE / WRITE_SIZE per dispatch (PMC runs)."""
+    import collections
+    import sqlite3
+    out = []
+    cur = sqlite3.connect(trace_db).cursor()
+    # rocprofv3's top_kernels view reports durations in microseconds.
+    out.append("| kernel | calls | total ms | avg ms | % |")
+    out.append("|---|---|---|---|---|")
+    for name, calls, total, avg, pct in cur.execute(
+            "select name, total_calls, total_duration, average, percentage from top_kernels"):
+        out.append(f"| `{name.split('(')[0]}` | {calls} | {total / 1e3:.1f} | {avg / 1e3:.3f} | {pct:.2f} |")
+    for label, db, counter in (("FETCH_SIZE", fetch_db, "FETCH_SIZE"), ("WRITE_SIZE", write_db, "WRITE_SIZE")):
+        if not db:
+            continue
+        c = sqlite3.connect(db).cursor()
+        agg = collections.defaultdict(list)
+        for name, v in c.execute("select kernel_name, value from counters_collection where counter_name=?",
+                                 (counter,)):
+            agg[name.split("(")[0]].append(v)
+        out.append("")
+        out.append(f"| kernel | dispatches | {label} KiB/dispatch (avg) |")
+        out.append("|---|---|---|")
+        for k, vs in agg.items():
+            out.append(f"| `{k}` | {len(vs)} | {sum(vs) / len(vs):.1f} |")
+    return "\n".join(out)
+
+
 def main() -> int:
+    if len(sys.argv) > 1 and sys.argv[1] == "--profile-summary":
+        args = sys.argv[2:] + [None, None]
+        print(profile_summary(args[0], args[1], args[2]))
+        return 0
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
@@ -93,6 +182,7 @@ def main() -> int:
     ap.add_argument("--layers", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--adjoint-batch", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -105,7 +195,8 @@ def main() -> int:
     log(f"bench: brickwork {n}q depth {args.layers} c128, {circ.count_gates()} gates")
     c128 = run_engine(n, "double", circ, args.steps, args.warmup)
     log(f"bench: c128 {c128['gates_per_s']:.1f} gates/s, {c128['hbm_gbs']:.0f} GB/s, "
-        f"kernel {c128['kernel_gbs']:.0f} GB/s")
+        f"kernel {c128['kernel_gbs']:.0f} GB/s, {c128['passes_per_step']:.0f} passes, "
+        f"{c128['gathers_per_step']:.0f} register layouts per step")
     extra = {}
     if not args.no_extra:
         c64 = run_engine(n, "single", circ, args.steps, args.warmup)
@@ -116,22 +207,16 @@ def main() -> int:
         q = circuits.qft(n)
 
         def gaussian(qv):
-            try:
-                import torch
-                g = torch.randn(1 << n, dtype=torch.complex128, device="cuda")
-                g /= torch.linalg.vector_norm(g)
-                torch.cuda.synchronize()
-                qv.initialize_from_device(g.data_ptr(), 1 << n, "double")
-                del g
-                torch.cuda.empty_cache()
-            except Exception as e:  # torch missing: a basis state still measures throughput
-                log(f"bench: QFT input from |0> ({e})")
-                qv.initialize()
+            qv.initialize_random(seed)  # i.i.d. complex Gaussian, normalised, on the device
         qres = run_engine(n, "double", q, max(1, args.steps), 1, init=gaussian)
         log(f"bench: qft {qres['gates_per_s']:.1f} gates/s")
         extra["qft_c128"] = {k: qres[k] for k in ("gates_per_s", "ms_per_step", "hbm_gbs",
                                                    "passes_per_step")}
         extra["qft_c128"]["gates"] = q.count_gates()
+        adj = run_adjoint(24, 12, args.adjoint_batch, 16, seed)
+        log(f"bench: adjoint 24q {adj['grads_per_s']:.0f} grads/s, param shift "
+            f"{adj['param_shift_grads_per_s']:.1f} grads/s ({adj['speedup_vs_param_shift']:.0f}x)")
+        extra["adjoint_24q_c128"] = adj
 
     cpu = None
     if not args.no_cpu:

@@ -116,6 +116,11 @@ int aerhip_qv_set_sample_measure_index_size(aerhip_qv *qv, int n);
 int aerhip_qv_initialize_from_device(aerhip_qv *qv, const void *src,
                                      uint64_t len, int src_precision);
 
+/* No counterpart in the reference (a test-input generator): fills the state
+ * with psi = g/|g|, g i.i.d. complex Gaussian from a counter-based hash of
+ * (seed, index), generated on the device. */
+int aerhip_qv_initialize_random(aerhip_qv *qv, uint64_t seed);
+
 /* No counterpart in the reference: flushes the queue and returns the device
  * address of the amplitudes (interop with device-side callers). */
 int aerhip_qv_device_pointer(aerhip_qv *qv, void **out);
@@ -126,8 +131,14 @@ int aerhip_qv_device_pointer(aerhip_qv *qv, void **out);
 int aerhip_qv_set_tile_bits(aerhip_qv *qv, int tile_bits, int low_bits);
 
 /* No counterpart in the reference: cumulative counters of the handle,
- * out3 = {passes, kernel launches of passes, queued gate primitives}. */
-int aerhip_qv_stats(aerhip_qv *qv, uint64_t *out3);
+ * out4 = {passes, kernel launches of passes, queued gate primitives,
+ * register layout changes inside tile passes}. */
+int aerhip_qv_stats(aerhip_qv *qv, uint64_t *out4);
+
+/* No counterpart in the reference: number of register qubits per thread in
+ * full tile passes (c128: 3 or 4, c64: 4 or 5; 0 keeps every gate on the LDS
+ * tile). */
+int aerhip_qv_set_register_bits(aerhip_qv *qv, int reg_bits);
 
 /* No counterpart in the reference (its timers are host wall clocks,
  * circuit_executor.hpp:715-723): brackets every flush's pass kernels with
@@ -217,6 +228,28 @@ int aerhip_expval_pauli(aerhip_qv *qv, const uint64_t *qubits, uint64_t nq,
                         const char *pauli, double phase_re, double phase_im,
                         double *out);
 
+/* Base::apply_save_expval for one Pauli string (src/simulators/state.hpp:
+ * 416-447, evaluated with QubitVector::expval_pauli): queues
+ * Re <psi| phase * P |psi> at the current point of the gate sequence, without
+ * a flush.  The engine evaluates it inside the tile pass that holds that
+ * point, so many saved values share the state sweeps of the gates (the GPU
+ * reference's batched_expval_pauli, qubitvector_thrust.hpp:2683-2713, also
+ * batches terms).  `slot_out` receives the value's index. */
+int aerhip_save_expval_pauli(aerhip_qv *qv, const uint64_t *qubits, uint64_t nq,
+                             const char *pauli, double phase_re, double phase_im,
+                             uint64_t *slot_out);
+
+/* Flushes and copies every saved value in slot order into `out`
+ * (`capacity` entries), writes their number to `count`, and clears them. */
+int aerhip_saved_expvals(aerhip_qv *qv, double *out, uint64_t capacity, uint64_t *count);
+
+/* No counterpart in the reference (it has no adjoint method): with the top
+ * qubit n-1 as a flag, writes sum_t coeffs[t] P_t psi into the half with the
+ * flag set, where psi is the half with the flag clear.  `labels` holds nterms
+ * strings of n-1 characters, character n-2-i acting on qubit i. */
+int aerhip_qv_pauli_sum_to_upper_half(aerhip_qv *qv, const char *labels, const double *coeffs,
+                                      uint64_t nterms);
+
 /* ---- circuit runner (controller slice) ---------------------------------- */
 
 /* Operation kinds of the circuit wire format (a subset of Aer's OpType,
@@ -283,6 +316,30 @@ int aerhip_run_circuit(const aerhip_op *ops, uint64_t num_ops,
                        uint64_t num_qubits, uint64_t num_memory,
                        const aerhip_run_config *cfg, uint64_t *memory_out,
                        double *statevector_out, double *timing_out);
+
+/* ---- adjoint differentiation -------------------------------------------- */
+
+/* No counterpart in the reference (which differentiates by re-running
+ * circuits, e.g. the parameter-shift rule through EstimatorV2,
+ * qiskit_aer/primitives/estimator_v2.py:112-159).  For each of `num_bindings`
+ * parameter rows, computes E = <psi|H|psi>, psi = U(theta)|0> on `num_qubits`
+ * qubits, and dE/dtheta by the adjoint method: one forward sweep, then one
+ * reverse sweep that uncomputes each gate on psi and on lambda = H psi and
+ * accumulates 2 Re <lambda| dU_k |psi_{k-1}> (no intermediate state stored).
+ *   ops          unitary operations (gates, matrices, diagonals, barriers);
+ *   param_index  per op, the parameter its first angle is bound to, or -1;
+ *                rx ry rz rxx ryy rzz rzx p u1 cp cu1 may be parametrized;
+ *   params       num_bindings x num_params, row major;
+ *   labels       nterms Pauli strings of num_qubits characters (character
+ *                num_qubits-1-i acts on qubit i), coeffs their real weights;
+ *   energy_out   num_bindings values, grad_out num_bindings x num_params.
+ * Uses one (num_qubits+1)-qubit statevector on `device`. */
+int aerhip_adjoint_expval_gradient(const aerhip_op *ops, uint64_t num_ops, uint64_t num_qubits,
+                                   const int64_t *param_index, uint64_t num_params,
+                                   uint64_t num_bindings, const double *params,
+                                   const char *labels, const double *coeffs, uint64_t nterms,
+                                   int precision, int device, double *energy_out,
+                                   double *grad_out);
 
 #ifdef __cplusplus
 }

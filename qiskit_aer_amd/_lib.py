@@ -86,9 +86,11 @@ _SIGNATURES = {
     "aerhip_qv_synchronize": (C.c_int, [C.c_void_p]),
     "aerhip_qv_set_sample_measure_index_size": (C.c_int, [C.c_void_p, C.c_int]),
     "aerhip_qv_initialize_from_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_int]),
-    "aerhip_qv_device_pointer": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "aerhip_qv_initialize_random": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "aerhip_qv_device_pointer":(C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "aerhip_qv_set_tile_bits": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "aerhip_qv_stats": (C.c_int, [C.c_void_p, _u64p]),
+    "aerhip_qv_set_register_bits": (C.c_int, [C.c_void_p, C.c_int]),
     "aerhip_qv_set_event_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "aerhip_qv_pass_timing": (C.c_int, [C.c_void_p, _dp, _u64p]),
     "aerhip_apply_matrix": (C.c_int, [C.c_void_p, _u64p, C.c_uint64, C.c_void_p]),
@@ -108,6 +110,14 @@ _SIGNATURES = {
     "aerhip_probabilities": (C.c_int, [C.c_void_p, _u64p, C.c_uint64, C.c_void_p]),
     "aerhip_sample_measure": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
     "aerhip_expval_pauli": (C.c_int, [C.c_void_p, _u64p, C.c_uint64, C.c_char_p, C.c_double, C.c_double, _dp]),
+    "aerhip_save_expval_pauli": (C.c_int, [C.c_void_p, _u64p, C.c_uint64, C.c_char_p, C.c_double,
+                                           C.c_double, _u64p]),
+    "aerhip_saved_expvals": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, _u64p]),
+    "aerhip_qv_pauli_sum_to_upper_half": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p, C.c_uint64]),
+    "aerhip_adjoint_expval_gradient": (C.c_int, [C.POINTER(AerhipOp), C.c_uint64, C.c_uint64,
+                                                 C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p,
+                                                 C.c_char_p, C.c_void_p, C.c_uint64, C.c_int,
+                                                 C.c_int, C.c_void_p, C.c_void_p]),
     "aerhip_run_config_default": (None, [C.POINTER(AerhipRunConfig)]),
     "aerhip_apply_ops": (C.c_int, [C.c_void_p, C.POINTER(AerhipOp), C.c_uint64]),
     "aerhip_run_circuit": (C.c_int, [C.POINTER(AerhipOp), C.c_uint64, C.c_uint64, C.c_uint64,

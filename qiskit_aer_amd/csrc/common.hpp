@@ -87,6 +87,7 @@ enum TileOpKind : uint32_t {
   K_MCPHASE = 6,  // multiply by phase where every control bit is 1
   K_PAULI = 7,    // Pauli string: flip xl, sign by parity(z), times phase
   K_PERM = 8,     // swap pairs of basis states of k local qubits
+  K_PROBE = 9,    // read-only: tile share of Re<psi|phase P|psi> into probe slot npairs
 };
 
 constexpr uint8_t kNotLocal = 0xFF;

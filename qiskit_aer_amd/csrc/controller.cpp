@@ -16,6 +16,7 @@
 #include <cstring>
 #include <map>
 #include <random>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
@@ -187,6 +188,110 @@ struct Runner {
     return meas;
   }
 };
+
+// An operation with storage of its own (for re-bound angles and inverses).
+struct OwnedOp {
+  int32_t type = AERHIP_OP_GATE;
+  std::string name;
+  std::vector<uint64_t> qubits;
+  std::vector<double> params;
+  std::string label;
+  aerhip_op view() const {
+    aerhip_op o;
+    std::memset(&o, 0, sizeof(o));
+    o.type = type;
+    o.name = name.c_str();
+    o.num_qubits = static_cast<uint32_t>(qubits.size());
+    o.qubits = qubits.data();
+    o.num_params = static_cast<uint32_t>(type == AERHIP_OP_GATE ? params.size() : params.size() / 2);
+    o.params = params.data();
+    o.string_param = label.empty() ? nullptr : label.c_str();
+    return o;
+  }
+};
+
+OwnedOp own(const aerhip_op &op) {
+  OwnedOp o;
+  o.type = op.type;
+  if (op.name) o.name = op.name;
+  o.qubits.assign(op.qubits, op.qubits + op.num_qubits);
+  const uint64_t np = op.type == AERHIP_OP_GATE ? op.num_params : 2ull * op.num_params;
+  if (np) o.params.assign(op.params, op.params + np);
+  if (op.string_param) o.label = op.string_param;
+  return o;
+}
+
+// The inverse of a unitary operation: rotations with negated angles, the
+// adjoint pairs s/sdg, t/tdg, sx/sxdg, self-inverse gates unchanged, matrices
+// conjugate-transposed and diagonals conjugated.
+OwnedOp inverse(const OwnedOp &op) {
+  OwnedOp r = op;
+  if (op.type == AERHIP_OP_MATRIX) {
+    const uint64_t D = 1ull << op.qubits.size();
+    for (uint64_t i = 0; i < D; ++i)
+      for (uint64_t j = 0; j < D; ++j) {
+        r.params[2 * (i + D * j)] = op.params[2 * (j + D * i)];
+        r.params[2 * (i + D * j) + 1] = -op.params[2 * (j + D * i) + 1];
+      }
+    return r;
+  }
+  if (op.type == AERHIP_OP_DIAGONAL) {
+    for (size_t i = 1; i < r.params.size(); i += 2) r.params[i] = -r.params[i];
+    return r;
+  }
+  if (op.type != AERHIP_OP_GATE) return r;
+  static const std::map<std::string, std::string> adjoint_name = {
+      {"s", "sdg"}, {"sdg", "s"}, {"t", "tdg"}, {"tdg", "t"}, {"sx", "sxdg"}, {"sxdg", "sx"},
+      {"csx", "csxdg"}, {"csxdg", "csx"}, {"mcsx", "mcsxdg"}, {"mcsxdg", "mcsx"}};
+  static const std::set<std::string> self_inverse = {
+      "id", "delay", "x", "y", "z", "h", "cx", "CX", "cy", "cz", "swap", "ccx", "ccz", "cswap",
+      "mcx", "mcy", "mcz", "mcswap", "mcx_gray", "ecr", "pauli"};
+  static const std::set<std::string> negate_first = {
+      "rx", "ry", "rz", "crx", "cry", "crz", "mcrx", "mcry", "mcrz", "rxx", "ryy", "rzz", "rzx",
+      "p", "u1", "cp", "cu1", "mcp", "mcphase", "mcu1", "r", "mcr"};
+  const std::string &n = op.name;
+  if (adjoint_name.count(n)) {
+    r.name = adjoint_name.at(n);
+  } else if (self_inverse.count(n)) {
+  } else if (negate_first.count(n)) {
+    r.params[0] = -op.params[0];
+  } else if (n == "u3" || n == "u" || n == "U" || n == "cu3" || n == "mcu3") {
+    r.params = {-op.params[0], -op.params[2], -op.params[1]};
+  } else if (n == "u2" || n == "cu2" || n == "mcu2") {
+    r.name = n == "u2" ? "u3" : (n == "cu2" ? "cu3" : "mcu3");
+    r.params = {-M_PI / 2., -op.params[1], -op.params[0]};
+  } else if (n == "cu" || n == "mcu") {
+    r.params = {-op.params[0], -op.params[2], -op.params[1], -op.params[3]};
+  } else {
+    throw std::invalid_argument("adjoint: no inverse for gate '" + n + "'");
+  }
+  return r;
+}
+
+// d/dtheta of a parametrized gate as sum_g w_g (-i/2) G_g U (U = exp(-i theta
+// G/2) for rotations; p = e^{i theta/2} rz; cp through its projector), so
+// that dE/dtheta = sum_g w_g Im <lambda| G_g |psi> after the gate.
+struct GenTerm {
+  std::vector<uint64_t> qubits;
+  std::string label;
+  double weight;
+};
+
+std::vector<GenTerm> generator(const OwnedOp &op) {
+  const std::string &n = op.name;
+  const auto &q = op.qubits;
+  if (op.type != AERHIP_OP_GATE) throw std::invalid_argument("adjoint: parameter on a non-gate op");
+  if (n == "rx") return {{{q[0]}, "X", 1.0}};
+  if (n == "ry") return {{{q[0]}, "Y", 1.0}};
+  if (n == "rz" || n == "p" || n == "u1") return {{{q[0]}, "Z", 1.0}};
+  if (n == "rxx") return {{{q[0], q[1]}, "XX", 1.0}};
+  if (n == "ryy") return {{{q[0], q[1]}, "YY", 1.0}};
+  if (n == "rzz") return {{{q[0], q[1]}, "ZZ", 1.0}};
+  if (n == "rzx") return {{{q[0], q[1]}, "XZ", 1.0}};
+  if (n == "cp" || n == "cu1")
+    return {{{q[0]}, "Z", 0.5}, {{q[1]}, "Z", 0.5}, {{q[0], q[1]}, "ZZ", -0.5}};
+  throw std::invalid_argument("adjoint: gate '" + n + "' cannot carry a differentiated parameter");
+}
 
 bool is_measure_tail(const aerhip_op *ops, uint64_t num_ops, uint64_t first) {
   for (uint64_t i = first; i < num_ops; ++i)
@@ -374,7 +479,7 @@ int aerhip_run_circuit(const aerhip_op *ops, uint64_t num_ops, uint64_t num_qubi
       t_gates = std::chrono::duration<double>(clk::now() - t0).count();
     }
     if (timing_out) {
-      uint64_t st[3] = {0, 0, 0};
+      uint64_t st[4] = {0, 0, 0, 0};
       ck(aerhip_qv_stats(R.qv, st));
       timing_out[0] = t_gates;
       timing_out[1] = t_sample;
@@ -417,6 +522,98 @@ int aerhip_apply_ops(aerhip_qv *qv, const aerhip_op *ops, uint64_t num_ops) {
         default:
           throw std::invalid_argument("aerhip_apply_ops: only gate, matrix, diagonal and barrier "
                                       "operations can be applied to a handle");
+      }
+    }
+    return 0;
+  } catch (const std::invalid_argument &e) {
+    aerhip_detail_set_error(e.what());
+    return 2;
+  } catch (const std::exception &e) {
+    aerhip_detail_set_error(e.what());
+    return 1;
+  }
+}
+
+// Energy and gradient of E(theta) = <psi(theta)| H |psi(theta)> by the
+// adjoint method (no counterpart in the reference, which differentiates by
+// re-running circuits, e.g. parameter shift through EstimatorV2).
+//
+// The state is held as Phi = |0>psi + |1>lambda on n+1 qubits (qubit n is a
+// flag): gates on qubits < n act on psi and lambda at once, and
+// <Phi| Y_n (x) G |Phi> = -2 Im <lambda|G|psi>, <Phi| X_n |Phi> = 2 Re <psi|lambda>.
+// One binding:
+//   1. Phi = |0> psi with psi = U(theta)|0>;  2. lambda = H psi (upper half);
+//   3. E = <Phi|X_n|Phi>/2;  4. for k = L..1: dE/dtheta_k from
+//      <Phi|Y_n (x) G_k|Phi> (queued, evaluated inside the tile passes), then
+//      apply U_k^dagger to both halves (uncomputation, nothing is stored).
+int aerhip_adjoint_expval_gradient(const aerhip_op *ops, uint64_t num_ops, uint64_t num_qubits,
+                                   const int64_t *param_index, uint64_t num_params,
+                                   uint64_t num_bindings, const double *params,
+                                   const char *labels, const double *coeffs, uint64_t nterms,
+                                   int precision, int device, double *energy_out,
+                                   double *grad_out) {
+  try {
+    if (num_ops && (!ops || !param_index)) throw std::invalid_argument("null op list");
+    if (num_bindings && num_params && !params) throw std::invalid_argument("null params");
+    if (nterms && (!labels || !coeffs)) throw std::invalid_argument("null Hamiltonian");
+    const uint64_t n = num_qubits;
+    std::vector<OwnedOp> base(num_ops);
+    for (uint64_t i = 0; i < num_ops; ++i) {
+      base[i] = own(ops[i]);
+      for (uint64_t q : base[i].qubits)
+        if (q >= n) throw std::invalid_argument("operation qubit out of range");
+      if (ops[i].type != AERHIP_OP_GATE && ops[i].type != AERHIP_OP_MATRIX &&
+          ops[i].type != AERHIP_OP_DIAGONAL && ops[i].type != AERHIP_OP_BARRIER)
+        throw std::invalid_argument("adjoint: only unitary operations are supported");
+      if (param_index[i] >= static_cast<int64_t>(num_params))
+        throw std::invalid_argument("adjoint: parameter index out of range");
+      if (param_index[i] >= 0 && base[i].params.empty())
+        throw std::invalid_argument("adjoint: parametrized op without an angle");
+    }
+    Runner R;
+    R.n = n + 1;
+    R.qv = aerhip_qv_create(n + 1, precision, device);
+    if (!R.qv) throw std::runtime_error(aerhip_last_error());
+    const uint64_t flag = n;
+    std::vector<double> vals;
+    for (uint64_t b = 0; b < num_bindings; ++b) {
+      std::vector<OwnedOp> bound = base;
+      for (uint64_t i = 0; i < num_ops; ++i)
+        if (param_index[i] >= 0) bound[i].params[0] = params[b * num_params + param_index[i]];
+      ck(aerhip_qv_initialize(R.qv));
+      for (const OwnedOp &o : bound) {
+        const aerhip_op v = o.view();
+        ck(aerhip_apply_ops(R.qv, &v, 1));
+      }
+      ck(aerhip_qv_pauli_sum_to_upper_half(R.qv, labels, coeffs, nterms));
+      uint64_t slot = 0;
+      ck(aerhip_save_expval_pauli(R.qv, &flag, 1, "X", 1.0, 0.0, &slot));
+      // (slot, parameter, weight) of every gradient probe
+      std::vector<std::tuple<uint64_t, int64_t, double>> probes;
+      for (uint64_t k = num_ops; k-- > 0;) {
+        const OwnedOp &o = bound[k];
+        if (o.type == AERHIP_OP_BARRIER) continue;
+        if (param_index[k] >= 0) {
+          for (const GenTerm &g : generator(o)) {
+            std::vector<uint64_t> q = g.qubits;
+            q.push_back(flag);
+            const std::string lab = "Y" + g.label;  // character 0 acts on the last qubit
+            ck(aerhip_save_expval_pauli(R.qv, q.data(), q.size(), lab.c_str(), 1.0, 0.0, &slot));
+            probes.emplace_back(slot, param_index[k], g.weight);
+          }
+        }
+        const OwnedOp inv = inverse(o);
+        const aerhip_op v = inv.view();
+        ck(aerhip_apply_ops(R.qv, &v, 1));
+      }
+      vals.assign(probes.size() + 1, 0.0);
+      uint64_t count = 0;
+      ck(aerhip_saved_expvals(R.qv, vals.data(), vals.size(), &count));
+      if (energy_out) energy_out[b] = 0.5 * vals[0];
+      if (grad_out) {
+        double *g = grad_out + b * num_params;
+        for (uint64_t p = 0; p < num_params; ++p) g[p] = 0.0;
+        for (const auto &t : probes) g[std::get<1>(t)] += std::get<2>(t) * (-0.5) * vals[std::get<0>(t)];
       }
     }
     return 0;

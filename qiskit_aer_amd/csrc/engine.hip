@@ -9,6 +9,7 @@
 #include "engine.hpp"
 #include "gate_matrices.hpp"
 #include "kernels.hpp"
+#include "tile_pass.hpp"
 
 namespace aerhip {
 
@@ -40,6 +41,8 @@ Engine::Engine(uint32_t num_qubits, int precision, int device)
   size_ = 1ull << n_;
   tile_bits_ = std::min<uint32_t>(n_, prec_ == AERHIP_PRECISION_DOUBLE ? kTileBitsC128 : kTileBitsC64);
   low_bits_ = std::min<uint32_t>(n_, prec_ == AERHIP_PRECISION_DOUBLE ? 3 : 4);
+  reg_bits_ = prec_ == AERHIP_PRECISION_DOUBLE ? 3 : 4;
+  if (const char *e = std::getenv("AERHIP_REG_BITS")) set_reg_bits(std::atoi(e));
   if (const char *e = std::getenv("AERHIP_LOW_BITS")) set_low_bits(std::atoi(e));
   AERHIP_CHECK(hipSetDevice(device_));
   const size_t bytes = size_ * (prec_ == AERHIP_PRECISION_DOUBLE ? 16 : 8);
@@ -56,6 +59,7 @@ Engine::~Engine() {
   if (d_prog_) (void)hipFree(d_prog_);
   if (d_partial_) (void)hipFree(d_partial_);
   if (d_scratch_) (void)hipFree(d_scratch_);
+  if (d_probes_) (void)hipFree(d_probes_);
   if (h_prog_) (void)hipHostFree(h_prog_);
   if (h_scalar_) (void)hipHostFree(h_scalar_);
   if (prog_ev_) (void)hipEventDestroy(prog_ev_);
@@ -86,8 +90,7 @@ void Engine::set_low_bits(uint32_t c) {
 // ---------------------------------------------------------------------------
 
 void Engine::initialize() {
-  queue_.clear();
-  pool_.clear();
+  discard_queue();
   AERHIP_CHECK(hipSetDevice(device_));
   if (prec_ == AERHIP_PRECISION_DOUBLE)
     k_fill_basis<double><<<grid_for(size_, 16384), kThreads, 0, stream_>>>(
@@ -99,8 +102,7 @@ void Engine::initialize() {
 }
 
 void Engine::zero() {
-  queue_.clear();
-  pool_.clear();
+  discard_queue();
   AERHIP_CHECK(hipSetDevice(device_));
   if (prec_ == AERHIP_PRECISION_DOUBLE)
     k_fill_basis<double><<<grid_for(size_, 16384), kThreads, 0, stream_>>>(
@@ -115,8 +117,7 @@ void Engine::initialize_from_vector(const cd *vec, uint64_t len) {
   if (len != size_)
     throw std::runtime_error("QubitVector::initialize input vector is incorrect length (" +
                              std::to_string(size_) + "!=" + std::to_string(len) + ")");
-  queue_.clear();
-  pool_.clear();
+  discard_queue();
   AERHIP_CHECK(hipSetDevice(device_));
   if (prec_ == AERHIP_PRECISION_DOUBLE) {
     AERHIP_CHECK(hipMemcpyAsync(sv_, vec, size_ * sizeof(cd), hipMemcpyHostToDevice, stream_));
@@ -139,8 +140,7 @@ void Engine::initialize_from_device(const void *src, uint64_t len, int src_preci
   if (len != size_)
     throw std::runtime_error("QubitVector::initialize input vector is incorrect length (" +
                              std::to_string(size_) + "!=" + std::to_string(len) + ")");
-  queue_.clear();
-  pool_.clear();
+  discard_queue();
   AERHIP_CHECK(hipSetDevice(device_));
   if (src_precision == prec_) {
     const size_t bytes = size_ * (prec_ == AERHIP_PRECISION_DOUBLE ? 16 : 8);
@@ -151,6 +151,28 @@ void Engine::initialize_from_device(const void *src, uint64_t len, int src_preci
   } else {
     k_convert<double, float><<<grid_for(size_, 16384), kThreads, 0, stream_>>>(
         static_cast<cx<double> *>(sv_), static_cast<const cx<float> *>(src), size_);
+  }
+  AERHIP_CHECK(hipGetLastError());
+  AERHIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+void Engine::initialize_random(uint64_t seed) {
+  discard_queue();
+  AERHIP_CHECK(hipSetDevice(device_));
+  const uint32_t grid = grid_for(size_, 4096);
+  ensure_partial(grid + 1);
+  if (prec_ == AERHIP_PRECISION_DOUBLE) {
+    auto *sv = static_cast<cx<double> *>(sv_);
+    k_gaussian<double><<<grid_for(size_, 16384), kThreads, 0, stream_>>>(sv, size_, seed);
+    k_sumsq<double><<<grid, kThreads, 0, stream_>>>(sv, size_, d_partial_);
+    k_sum_rows<<<1, kThreads, 0, stream_>>>(d_partial_, 1, grid, d_partial_ + grid);
+    k_scale<double><<<grid_for(size_, 16384), kThreads, 0, stream_>>>(sv, size_, d_partial_ + grid);
+  } else {
+    auto *sv = static_cast<cx<float> *>(sv_);
+    k_gaussian<float><<<grid_for(size_, 16384), kThreads, 0, stream_>>>(sv, size_, seed);
+    k_sumsq<float><<<grid, kThreads, 0, stream_>>>(sv, size_, d_partial_);
+    k_sum_rows<<<1, kThreads, 0, stream_>>>(d_partial_, 1, grid, d_partial_ + grid);
+    k_scale<float><<<grid_for(size_, 16384), kThreads, 0, stream_>>>(sv, size_, d_partial_ + grid);
   }
   AERHIP_CHECK(hipGetLastError());
   AERHIP_CHECK(hipStreamSynchronize(stream_));
@@ -207,10 +229,27 @@ void Engine::push(Prim &&p) {
   for (uint32_t q : p.q) p.touch |= bit(q);
   for (uint32_t q : p.ctrl) p.touch |= bit(q);
   p.touch |= p.xmask | p.zmask;
+  // A non-unitary operator does not commute with an expectation value on
+  // other qubits (it rescales them), so nothing is reordered across it.
+  if (!p.unitary) p.touch = ~0ull;
+  if (p.kind != K_PROBE) stats_.gates++;
   queue_.push_back(std::move(p));
-  stats_.gates++;
   if (queue_.size() >= kMaxQueue) flush();
 }
+
+namespace {
+
+bool is_unitary(const cd *m, uint64_t D) {
+  for (uint64_t i = 0; i < D; ++i)
+    for (uint64_t j = 0; j < D; ++j) {
+      cd s = 0;
+      for (uint64_t r = 0; r < D; ++r) s += std::conj(m[r + D * i]) * m[r + D * j];
+      if (std::abs(s - (i == j ? 1.0 : 0.0)) > 1e-10) return false;
+    }
+  return true;
+}
+
+}  // namespace
 
 void Engine::push_dense(const std::vector<uint32_t> &targets, const std::vector<uint32_t> &ctrl,
                         uint64_t ctrl_val, const cd *mat) {
@@ -224,6 +263,7 @@ void Engine::push_dense(const std::vector<uint32_t> &targets, const std::vector<
   p.off = pool_add(mat, p.len);
   for (uint32_t q : targets) p.tmask |= bit(q);
   p.solo = k > 5;
+  p.unitary = k <= 5 && is_unitary(mat, 1ull << k);
   push(std::move(p));
 }
 
@@ -238,6 +278,8 @@ void Engine::push_diag(const std::vector<uint32_t> &qubits, const std::vector<ui
   p.ctrl_val = ctrl_val;
   p.len = static_cast<uint32_t>(1ull << qubits.size());
   p.off = pool_add(diag, p.len);
+  for (uint32_t i = 0; i < p.len; ++i)
+    if (std::abs(std::abs(diag[i]) - 1.0) > 1e-12) p.unitary = false;
   push(std::move(p));
 }
 
@@ -533,6 +575,106 @@ void Engine::initialize_component(const std::vector<uint32_t> &qubits,
 // and so does every later primitive that touches their qubits.
 // ---------------------------------------------------------------------------
 
+// Rewrites the LDS-form op list of a full tile pass into a register program:
+// 1-qubit matrices, 2-qubit matrices, X and Y gates whose mixed qubits are
+// the current register qubits run in registers; diagonals and phases run in
+// either form; everything else runs on the LDS tile.  A GATHER picks the
+// register qubits: the mixed qubits of the next gate plus those of the gates
+// that follow it, as long as they fit, then the highest free tile bits
+// (consecutive threads then read consecutive LDS words).
+std::vector<TileOp> register_program(const std::vector<TileOp> &in, uint32_t nb, uint32_t R,
+                                     uint64_t *gathers) {
+  // 0: runs in registers if its mixed qubits are register qubits, 1: runs in
+  // any form, 2: LDS only.
+  auto classify = [](const TileOp &op, uint32_t &tm) -> int {
+    switch (op.kind) {
+      case K_DENSE1:
+      case K_MCX:
+      case K_MCY:
+        tm = 1u << op.t;
+        return 0;
+      case K_DENSE:
+        if (op.k == 2) {
+          tm = (1u << op.pos[0]) | (1u << op.pos[1]);
+          return 0;
+        }
+        return 2;
+      case K_DIAG:
+      case K_MCPHASE:
+        tm = 0;
+        return 1;
+      default:
+        return 2;
+    }
+  };
+  auto regidx = [](uint32_t regset, uint32_t b) -> uint32_t {
+    return static_cast<uint32_t>(__builtin_popcount(regset & ((1u << b) - 1u)));
+  };
+  std::vector<TileOp> out;
+  out.reserve(in.size() * 2);
+  bool inreg = false;
+  uint32_t regset = 0;
+  TileOp scatter;
+  std::memset(&scatter, 0, sizeof(scatter));
+  scatter.kind = K_SCATTER;
+  for (size_t i = 0; i < in.size(); ++i) {
+    TileOp op = in[i];
+    uint32_t tm = 0;
+    const int cls = classify(op, tm);
+    if (cls == 0) {
+      if (!inreg || (tm & ~regset)) {
+        uint32_t rs = tm;
+        for (size_t j = i + 1; j < in.size(); ++j) {
+          uint32_t tj = 0;
+          const int cj = classify(in[j], tj);
+          if (cj == 1) continue;
+          if (cj == 2) break;
+          const uint32_t u = rs | tj;
+          if (static_cast<uint32_t>(__builtin_popcount(u)) > R) break;
+          rs = u;
+        }
+        for (int b = static_cast<int>(nb) - 1; b >= 0 && static_cast<uint32_t>(__builtin_popcount(rs)) < R; --b)
+          rs |= 1u << b;
+        if (inreg) out.push_back(scatter);
+        TileOp g;
+        std::memset(&g, 0, sizeof(g));
+        g.kind = K_GATHER;
+        uint32_t pr = 0, pt = R;
+        for (uint32_t b = 0; b < nb; ++b) {
+          if (rs & (1u << b)) g.pos[pr++] = static_cast<uint8_t>(b);
+          else g.pos[pt++] = static_cast<uint8_t>(b);
+        }
+        out.push_back(g);
+        if (gathers) ++*gathers;
+        regset = rs;
+        inreg = true;
+      }
+      switch (op.kind) {
+        case K_DENSE1: op.kind = K_RDENSE1; op.t = regidx(regset, op.t); break;
+        case K_MCX: op.kind = K_RMCX; op.t = regidx(regset, op.t); break;
+        case K_MCY: op.kind = K_RMCY; op.t = regidx(regset, op.t); break;
+        default:
+          op.kind = K_RDENSE2;
+          op.pos[0] = static_cast<uint8_t>(regidx(regset, op.pos[0]));
+          op.pos[1] = static_cast<uint8_t>(regidx(regset, op.pos[1]));
+          break;
+      }
+      out.push_back(op);
+    } else if (cls == 1) {
+      if (inreg) op.kind = op.kind == K_DIAG ? K_RDIAG : K_RMCPHASE;
+      out.push_back(op);
+    } else {
+      if (inreg) {
+        out.push_back(scatter);
+        inreg = false;
+      }
+      out.push_back(op);
+    }
+  }
+  if (inreg) out.push_back(scatter);
+  return out;
+}
+
 std::vector<PassPlan> Engine::plan() const {
   std::vector<PassPlan> out;
   const uint32_t nb = tile_bits_;
@@ -614,7 +756,7 @@ void Engine::flush() {
     pd.c = c;
     pd.ncomp = nc;
     pd.op_begin = static_cast<uint32_t>(ops.size());
-    pd.nops = static_cast<uint32_t>(pp.prims.size());
+    std::vector<TileOp> pops;
     for (uint32_t idx : pp.prims) {
       const Prim &p = queue_[idx];
       TileOp op;
@@ -647,7 +789,8 @@ void Engine::flush() {
         case K_MCY:
           op.t = op.pos[0];
           break;
-        case K_PAULI: {
+        case K_PAULI:
+        case K_PROBE: {
           for (uint32_t q = 0; q < n_; ++q) {
             if (p.xmask & bit(q)) op.xl |= 1u << loc[q];
             if (p.zmask & bit(q)) {
@@ -662,8 +805,16 @@ void Engine::flush() {
         default:
           break;
       }
-      ops.push_back(op);
+      pops.push_back(op);
     }
+    const uint32_t full_bits =
+        prec_ == AERHIP_PRECISION_DOUBLE ? kTileBitsC128 : kTileBitsC64;
+    if (!pp.solo && reg_bits_ > 0 && ns == full_bits) {
+      pd.kind = 2;  // register-blocked tile pass
+      pops = register_program(pops, ns, reg_bits_, &stats_.gathers);
+    }
+    pd.nops = static_cast<uint32_t>(pops.size());
+    ops.insert(ops.end(), pops.begin(), pops.end());
     passes.push_back(pd);
   }
 
@@ -696,17 +847,52 @@ void Engine::flush() {
   const PassDesc *dpass = reinterpret_cast<const PassDesc *>(d);
   const TileOp *dops = reinterpret_cast<const TileOp *>(d + ops_off);
   const cx<double> *dpool = reinterpret_cast<const cx<double> *>(d + pool_off);
+  // Probe partials: one double per (probe, tile); every tile pass has
+  // 2^(n - tile_bits) tiles.
+  double *dprobes = nullptr;
+  const uint64_t nprobes = probes_queued_;
+  const uint64_t ntiles = 1ull << (n_ - tile_bits_);
+  if (nprobes) {
+    const size_t need = (nprobes * ntiles + nprobes) * sizeof(double);
+    if (need > probes_cap_) {
+      AERHIP_CHECK(hipStreamSynchronize(stream_));
+      if (d_probes_) AERHIP_CHECK(hipFree(d_probes_));
+      probes_cap_ = align_up(need, 1 << 20);
+      AERHIP_CHECK(hipMalloc(reinterpret_cast<void **>(&d_probes_), probes_cap_));
+    }
+    dprobes = d_probes_;
+  }
   if (event_timing_) AERHIP_CHECK(hipEventRecord(t0_ev_, stream_));
   for (size_t i = 0; i < passes.size(); ++i) {
     const PassDesc &pd = passes[i];
-    if (pd.kind == 0) {
+    if (pd.kind == 2) {
+      const uint32_t grid = static_cast<uint32_t>(1ull << pd.ncomp);
+      if (prec_ == AERHIP_PRECISION_DOUBLE) {
+        auto *sv = static_cast<cx<double> *>(sv_);
+        if (reg_bits_ == 3)
+          k_tile_reg<double, kTileBitsC128, 3><<<grid, 1 << (kTileBitsC128 - 3), 0, stream_>>>(
+              sv, dpass + i, dops, dpool, dprobes);
+        else
+          k_tile_reg<double, kTileBitsC128, 4><<<grid, 1 << (kTileBitsC128 - 4), 0, stream_>>>(
+              sv, dpass + i, dops, dpool, dprobes);
+      } else {
+        auto *sv = static_cast<cx<float> *>(sv_);
+        if (reg_bits_ == 4)
+          k_tile_reg<float, kTileBitsC64, 4><<<grid, 1 << (kTileBitsC64 - 4), 0, stream_>>>(
+              sv, dpass + i, dops, dpool, dprobes);
+        else
+          k_tile_reg<float, kTileBitsC64, 5><<<grid, 1 << (kTileBitsC64 - 5), 0, stream_>>>(
+              sv, dpass + i, dops, dpool, dprobes);
+      }
+    } else if (pd.kind == 0) {
+      // LDS-resident tile pass: the same kernel without register layouts.
       const uint32_t grid = static_cast<uint32_t>(1ull << pd.ncomp);
       if (prec_ == AERHIP_PRECISION_DOUBLE)
-        k_tile_pass<double, kTileBitsC128><<<grid, kThreads, 0, stream_>>>(
-            static_cast<cx<double> *>(sv_), dpass + i, dops, dpool);
+        k_tile_reg<double, kTileBitsC128, 3><<<grid, kThreads, 0, stream_>>>(
+            static_cast<cx<double> *>(sv_), dpass + i, dops, dpool, dprobes);
       else
-        k_tile_pass<float, kTileBitsC64><<<grid, kThreads, 0, stream_>>>(
-            static_cast<cx<float> *>(sv_), dpass + i, dops, dpool);
+        k_tile_reg<float, kTileBitsC64, 4><<<grid, kThreads, 0, stream_>>>(
+            static_cast<cx<float> *>(sv_), dpass + i, dops, dpool, dprobes);
     } else {
       const uint32_t grid = static_cast<uint32_t>(1ull << (n_ - pd.nb));
       if (prec_ == AERHIP_PRECISION_DOUBLE)
@@ -720,6 +906,19 @@ void Engine::flush() {
     stats_.launches++;
   }
   stats_.passes += passes.size();
+  if (nprobes) {
+    // Sum each probe's tile partials in tile order and fetch the values.
+    double *sums = d_probes_ + nprobes * ntiles;
+    k_sum_rows<<<grid_for(nprobes, 1u << 30), kThreads, 0, stream_>>>(
+        d_probes_, nprobes, static_cast<uint32_t>(ntiles), sums);
+    AERHIP_CHECK(hipGetLastError());
+    const size_t base = saved_expvals_.size();
+    saved_expvals_.resize(base + nprobes);
+    AERHIP_CHECK(hipMemcpyAsync(saved_expvals_.data() + base, sums, nprobes * sizeof(double),
+                                hipMemcpyDeviceToHost, stream_));
+    AERHIP_CHECK(hipStreamSynchronize(stream_));
+    probes_queued_ = 0;
+  }
   if (event_timing_) {
     AERHIP_CHECK(hipEventRecord(t1_ev_, stream_));
     AERHIP_CHECK(hipEventSynchronize(t1_ev_));
@@ -730,6 +929,13 @@ void Engine::flush() {
   }
   queue_.clear();
   pool_.clear();
+}
+
+void Engine::set_reg_bits(uint32_t r) {
+  const bool ok = r == 0 || (prec_ == AERHIP_PRECISION_DOUBLE ? (r == 3 || r == 4) : (r == 4 || r == 5));
+  if (!ok) throw std::invalid_argument("aerhip: register bits must be 0, 3 or 4 (c128), 0, 4 or 5 (c64)");
+  flush();
+  reg_bits_ = r;
 }
 
 void Engine::set_event_timing(bool on) {
@@ -931,6 +1137,69 @@ double Engine::expval_t(uint64_t xmask, uint64_t zmask, uint32_t xmax, cd ph) {
   return fetch_scalar(d_partial_ + grid);
 }
 
+// Pending gates are dropped when the state is overwritten, but saved
+// expectation values are evaluated first.
+void Engine::discard_queue() {
+  if (probes_queued_) flush();
+  queue_.clear();
+  pool_.clear();
+}
+
+uint64_t Engine::save_expval_pauli(const std::vector<uint32_t> &qubits, const std::string &pauli,
+                                   cd phase) {
+  check_qubits(qubits);
+  const PauliMasks m = pauli_masks(qubits, pauli);
+  const uint64_t slot = saved_expvals_.size() + probes_queued_;
+  Prim p;
+  p.kind = K_PROBE;
+  const cd ph = add_y_phase(m.num_y, phase);
+  p.pr = ph.real();
+  p.pi = ph.imag();
+  p.xmask = m.x;
+  p.zmask = m.z;
+  p.tmask = m.x;
+  if (m.x) p.q = {m.x_max};
+  p.npairs = static_cast<uint32_t>(probes_queued_++);
+  push(std::move(p));
+  return slot;
+}
+
+std::vector<double> Engine::take_saved_expvals() {
+  flush();
+  std::vector<double> out;
+  out.swap(saved_expvals_);
+  return out;
+}
+
+void Engine::pauli_sum_to_upper(const std::vector<std::string> &labels,
+                                const std::vector<double> &coeffs) {
+  if (n_ < 2) throw std::invalid_argument("aerhip: pauli_sum_to_upper needs two or more qubits");
+  if (labels.size() != coeffs.size()) throw std::invalid_argument("aerhip: labels/coeffs size");
+  std::vector<uint32_t> lower(n_ - 1);
+  for (uint32_t q = 0; q + 1 < n_; ++q) lower[q] = q;
+  std::vector<PauliTerm> terms;
+  for (size_t t = 0; t < labels.size(); ++t) {
+    const PauliMasks m = pauli_masks(lower, labels[t]);
+    const cd c = add_y_phase(m.num_y, cd(coeffs[t], 0.0));
+    terms.push_back(PauliTerm{m.x, m.z, c.real(), c.imag()});
+  }
+  std::stable_sort(terms.begin(), terms.end(),
+                   [](const PauliTerm &a, const PauliTerm &b) { return a.x < b.x; });
+  flush();
+  AERHIP_CHECK(hipSetDevice(device_));
+  PauliTerm *dt = static_cast<PauliTerm *>(scratch(std::max<size_t>(1, terms.size()) * sizeof(PauliTerm)));
+  AERHIP_CHECK(hipMemcpy(dt, terms.data(), terms.size() * sizeof(PauliTerm), hipMemcpyHostToDevice));
+  const uint64_t half = size_ >> 1;
+  if (prec_ == AERHIP_PRECISION_DOUBLE)
+    k_pauli_sum_upper<double><<<grid_for(half, 16384), kThreads, 0, stream_>>>(
+        static_cast<cx<double> *>(sv_), half, dt, static_cast<uint32_t>(terms.size()));
+  else
+    k_pauli_sum_upper<float><<<grid_for(half, 16384), kThreads, 0, stream_>>>(
+        static_cast<cx<float> *>(sv_), half, dt, static_cast<uint32_t>(terms.size()));
+  AERHIP_CHECK(hipGetLastError());
+  AERHIP_CHECK(hipStreamSynchronize(stream_));
+}
+
 double Engine::expval_pauli(const std::vector<uint32_t> &qubits, const std::string &pauli, cd phase) {
   check_qubits(qubits);
   const PauliMasks m = pauli_masks(qubits, pauli);
@@ -1060,6 +1329,10 @@ int aerhip_qv_initialize_from_device(aerhip_qv *qv, const void *src, uint64_t le
   });
 }
 
+int aerhip_qv_initialize_random(aerhip_qv *qv, uint64_t seed) {
+  return guard([&] { E(qv).initialize_random(seed); });
+}
+
 int aerhip_qv_device_pointer(aerhip_qv *qv, void **out) {
   return guard([&] {
     if (!out) throw std::invalid_argument("aerhip: null argument");
@@ -1093,13 +1366,47 @@ int aerhip_qv_set_tile_bits(aerhip_qv *qv, int tile_bits, int low_bits) {
   });
 }
 
-int aerhip_qv_stats(aerhip_qv *qv, uint64_t *out3) {
+int aerhip_qv_stats(aerhip_qv *qv, uint64_t *out3) {  // 4 values
   return guard([&] {
     if (!out3) throw std::invalid_argument("aerhip: null argument");
     out3[0] = E(qv).stats().passes;
     out3[1] = E(qv).stats().launches;
     out3[2] = E(qv).stats().gates;
+    out3[3] = E(qv).stats().gathers;
   });
+}
+
+int aerhip_save_expval_pauli(aerhip_qv *qv, const uint64_t *qubits, uint64_t nq,
+                             const char *pauli, double phase_re, double phase_im,
+                             uint64_t *slot_out) {
+  return guard([&] {
+    if (!pauli) throw std::invalid_argument("aerhip: null Pauli label");
+    const uint64_t slot = E(qv).save_expval_pauli(qvec(qubits, nq), pauli, cd(phase_re, phase_im));
+    if (slot_out) *slot_out = slot;
+  });
+}
+
+int aerhip_saved_expvals(aerhip_qv *qv, double *out, uint64_t capacity, uint64_t *count) {
+  return guard([&] {
+    const auto v = E(qv).take_saved_expvals();
+    if (count) *count = v.size();
+    if (v.size() > capacity) throw std::invalid_argument("aerhip: output buffer too small");
+    if (!v.empty()) std::memcpy(out, v.data(), v.size() * sizeof(double));
+  });
+}
+
+int aerhip_qv_pauli_sum_to_upper_half(aerhip_qv *qv, const char *labels, const double *coeffs,
+                                      uint64_t nterms) {
+  return guard([&] {
+    const uint64_t w = E(qv).num_qubits() - 1;
+    std::vector<std::string> labs(nterms);
+    for (uint64_t t = 0; t < nterms; ++t) labs[t].assign(labels + t * w, w);
+    E(qv).pauli_sum_to_upper(labs, std::vector<double>(coeffs, coeffs + nterms));
+  });
+}
+
+int aerhip_qv_set_register_bits(aerhip_qv *qv, int reg_bits) {
+  return guard([&] { E(qv).set_reg_bits(static_cast<uint32_t>(reg_bits)); });
 }
 
 int aerhip_qv_set_event_timing(aerhip_qv *qv, int enable) {

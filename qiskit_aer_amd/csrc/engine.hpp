@@ -27,6 +27,7 @@ struct Prim {
   uint64_t tmask = 0;          // qubits the primitive mixes (must be tile qubits)
   uint64_t touch = 0;          // every qubit it reads
   bool solo = false;           // runs as its own dense-group pass
+  bool unitary = true;         // false for projectors and other non-unitary maps
 };
 
 struct PassPlan {
@@ -41,6 +42,7 @@ struct EngineStats {
   uint64_t gates = 0;
   double pass_ms = 0;          // device time of the pass kernels (event timing on)
   uint64_t timed_launches = 0; // launches covered by pass_ms
+  uint64_t gathers = 0;        // register layout changes in tile passes
 };
 
 class Engine {
@@ -62,6 +64,7 @@ class Engine {
   void zero();
   void initialize_from_vector(const cd *vec, uint64_t len);
   void initialize_from_device(const void *src, uint64_t len, int src_precision);
+  void initialize_random(uint64_t seed);
   void copy_to_vector(cd *out, uint64_t len);
   void initialize_component(const std::vector<uint32_t> &qubits, const std::vector<cd> &state);
   void synchronize();
@@ -69,6 +72,7 @@ class Engine {
   void set_tile_bits(uint32_t nb);
   void set_low_bits(uint32_t c);
   void set_event_timing(bool on);
+  void set_reg_bits(uint32_t r);
 
   // gates (queued)
   void apply_matrix(const std::vector<uint32_t> &qubits, const std::vector<cd> &mat);
@@ -93,11 +97,23 @@ class Engine {
   std::vector<uint64_t> sample_measure(const std::vector<double> &rnds);
   double expval_pauli(const std::vector<uint32_t> &qubits, const std::string &pauli, cd phase);
 
+  // Queues Re(<psi| phase * P |psi>) at the current point of the gate
+  // sequence (evaluated inside the tile passes); returns its slot.
+  uint64_t save_expval_pauli(const std::vector<uint32_t> &qubits, const std::string &pauli,
+                             cd phase);
+  // Flushes and returns every saved value in slot order, then clears them.
+  std::vector<double> take_saved_expvals();
+  // With the top qubit as a flag: writes sum_t c_t P_t psi (psi = the half
+  // with the top qubit 0) into the half with the top qubit 1.  Labels act on
+  // qubits 0..n-2 (character n-2-i on qubit i).
+  void pauli_sum_to_upper(const std::vector<std::string> &labels, const std::vector<double> &coeffs);
+
   // Executes the queue.
   void flush();
 
  private:
   void check_qubits(const std::vector<uint32_t> &qubits) const;
+  void discard_queue();
   void push(Prim &&p);
   uint32_t pool_add(const cd *v, size_t n);
   void push_dense(const std::vector<uint32_t> &targets, const std::vector<uint32_t> &ctrl,
@@ -124,6 +140,7 @@ class Engine {
   int sample_index_size_ = 10;
   uint32_t tile_bits_;  // largest tile (11 for c128, 12 for c64: 32 KiB of LDS)
   uint32_t low_bits_;   // qubits every tile holds (one 128-byte run)
+  uint32_t reg_bits_;   // register qubits per thread in full tiles (0: LDS-only passes)
 
   std::vector<Prim> queue_;
   std::vector<cd> pool_;
@@ -140,6 +157,11 @@ class Engine {
   double *h_scalar_ = nullptr;  // pinned
   void *d_scratch_ = nullptr;   // descriptors, draws and samples of readouts
   size_t scratch_cap_ = 0;
+
+  uint64_t probes_queued_ = 0;        // probes in the current queue
+  std::vector<double> saved_expvals_; // values of flushed probes, slot order
+  double *d_probes_ = nullptr;        // per-tile probe partials
+  size_t probes_cap_ = 0;
 
   EngineStats stats_;
   bool event_timing_ = false;

@@ -23,219 +23,6 @@ __device__ __forceinline__ uint64_t insert_zero64(uint64_t p, uint32_t t) {
 }
 
 // ---------------------------------------------------------------------------
-// Tile pass.
-// BMAX is the largest tile (2^BMAX amplitudes, 32 KiB of LDS for both
-// precisions); RMAX amplitudes per thread.
-// ---------------------------------------------------------------------------
-template <typename T, int BMAX>
-__global__ __launch_bounds__(kThreads) void k_tile_pass(
-    cx<T> *__restrict__ sv, const PassDesc *__restrict__ pd,
-    const TileOp *__restrict__ ops, const cx<double> *__restrict__ pool) {
-  constexpr int RMAX = (1 << BMAX) / kThreads;
-  __shared__ cx<T> tile[1 << BMAX];
-  __shared__ uint64_t hioff[256];
-  __shared__ cx<T> mat[1024];
-  __shared__ uint32_t doff[32];
-
-  const uint32_t tid = threadIdx.x;
-  const uint32_t nb = pd->nb;
-  const uint32_t c = pd->c;
-  const uint32_t N = 1u << nb;
-  const uint32_t ncomp = pd->ncomp;
-
-  uint64_t base = 0;
-  const uint64_t tile_id = blockIdx.x;
-  for (uint32_t i = 0; i < ncomp; ++i)
-    base |= ((tile_id >> i) & 1ull) << pd->comp[i];
-
-  const uint32_t nh = nb - c;
-  for (uint32_t h = tid; h < (1u << nh); h += kThreads) {
-    uint64_t o = 0;
-    for (uint32_t j = 0; j < nh; ++j)
-      o |= (uint64_t)((h >> j) & 1u) << pd->spos[c + j];
-    hioff[h] = o;
-  }
-  __syncthreads();
-  const uint32_t cmask = (1u << c) - 1u;
-  for (uint32_t l = tid; l < N; l += kThreads)
-    tile[l] = sv[base + (l & cmask) + hioff[l >> c]];
-
-  const uint32_t nops = pd->nops;
-  const TileOp *opl = ops + pd->op_begin;
-  for (uint32_t o = 0; o < nops; ++o) {
-    const TileOp &op = opl[o];
-    if ((base & op.cg) != op.cvg) continue;  // uniform over the workgroup
-    __syncthreads();
-    const uint32_t cl = op.cl, cvl = op.cvl;
-    switch (op.kind) {
-      case K_DENSE1: {
-        const cx<T> m0 = to_t<T>(pool[op.off + 0]);
-        const cx<T> m1 = to_t<T>(pool[op.off + 1]);
-        const cx<T> m2 = to_t<T>(pool[op.off + 2]);
-        const cx<T> m3 = to_t<T>(pool[op.off + 3]);
-        const uint32_t t = op.t;
-        for (uint32_t p = tid; p < (N >> 1); p += kThreads) {
-          const uint32_t l0 = insert_zero(p, t);
-          const uint32_t l1 = l0 | (1u << t);
-          if ((l0 & cl) != cvl) continue;
-          const cx<T> a0 = tile[l0];
-          const cx<T> a1 = tile[l1];
-          tile[l0] = cfma(m2, a1, cmul(m0, a0));
-          tile[l1] = cfma(m3, a1, cmul(m1, a0));
-        }
-        break;
-      }
-      case K_DENSE: {
-        const uint32_t k = op.k;
-        const uint32_t D = 1u << k;
-        uint32_t tmask = 0;
-        for (uint32_t b = 0; b < k; ++b) tmask |= 1u << op.pos[b];
-        for (uint32_t e = tid; e < D * D; e += kThreads) mat[e] = to_t<T>(pool[op.off + e]);
-        if (tid < D) {
-          uint32_t d = 0;
-          for (uint32_t b = 0; b < k; ++b) d |= ((tid >> b) & 1u) << op.pos[b];
-          doff[tid] = d;
-        }
-        __syncthreads();
-        cx<T> out[RMAX];
-#pragma unroll
-        for (int r = 0; r < RMAX; ++r) {
-          const uint32_t l = tid + r * kThreads;
-          if (l < N) {
-            if ((l & cl) != cvl) {
-              out[r] = tile[l];
-            } else {
-              uint32_t i = 0;
-              for (uint32_t b = 0; b < k; ++b) i |= ((l >> op.pos[b]) & 1u) << b;
-              const uint32_t lb = l & ~tmask;
-              cx<T> acc = mk<T>(0, 0);
-              for (uint32_t j = 0; j < D; ++j) acc = cfma(mat[i + D * j], tile[lb | doff[j]], acc);
-              out[r] = acc;
-            }
-          }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < RMAX; ++r) {
-          const uint32_t l = tid + r * kThreads;
-          if (l < N) tile[l] = out[r];
-        }
-        break;
-      }
-      case K_DIAG: {
-        const uint32_t k = op.k;
-        uint32_t ivb = 0;
-        for (uint32_t j = 0; j < k; ++j)
-          if (op.pos[j] == kNotLocal) ivb |= (uint32_t)((base >> op.gpos[j]) & 1ull) << j;
-        for (uint32_t l = tid; l < N; l += kThreads) {
-          if ((l & cl) != cvl) continue;
-          uint32_t iv = ivb;
-          for (uint32_t j = 0; j < k; ++j)
-            if (op.pos[j] != kNotLocal) iv |= ((l >> op.pos[j]) & 1u) << j;
-          tile[l] = cmul(tile[l], to_t<T>(pool[op.off + iv]));
-        }
-        break;
-      }
-      case K_MCX:
-      case K_MCY: {
-        const uint32_t t = op.t;
-        const bool isy = op.kind == K_MCY;
-        for (uint32_t p = tid; p < (N >> 1); p += kThreads) {
-          const uint32_t l0 = insert_zero(p, t);
-          const uint32_t l1 = l0 | (1u << t);
-          if ((l0 & cl) != cvl) continue;
-          const cx<T> a0 = tile[l0];
-          const cx<T> a1 = tile[l1];
-          if (isy) {
-            tile[l0] = mk<T>(a1.y, -a1.x);  // -i * a1
-            tile[l1] = mk<T>(-a0.y, a0.x);  //  i * a0
-          } else {
-            tile[l0] = a1;
-            tile[l1] = a0;
-          }
-        }
-        break;
-      }
-      case K_MCSWAP: {
-        const uint32_t pa = op.pos[0], pb = op.pos[1];
-        const uint32_t lo = pa < pb ? pa : pb, hi = pa < pb ? pb : pa;
-        for (uint32_t p = tid; p < (N >> 2); p += kThreads) {
-          const uint32_t l = insert_zero(insert_zero(p, lo), hi);
-          if ((l & cl) != cvl) continue;
-          const uint32_t la = l | (1u << pa), lb = l | (1u << pb);
-          const cx<T> va = tile[la];
-          tile[la] = tile[lb];
-          tile[lb] = va;
-        }
-        break;
-      }
-      case K_MCPHASE: {
-        const cx<T> ph = mk<T>(static_cast<T>(op.pr), static_cast<T>(op.pi));
-        for (uint32_t l = tid; l < N; l += kThreads)
-          if ((l & cl) == cvl) tile[l] = cmul(tile[l], ph);
-        break;
-      }
-      case K_PAULI: {
-        const cx<T> ph = mk<T>(static_cast<T>(op.pr), static_cast<T>(op.pi));
-        const uint32_t sgb = __popcll(base & op.zg) & 1u;
-        const uint32_t xl = op.xl, zl = op.zl;
-        if (xl == 0) {
-          for (uint32_t l = tid; l < N; l += kThreads) {
-            cx<T> v = tile[l];
-            if ((__popc(l & zl) & 1u) ^ sgb) v = cneg(v);
-            tile[l] = cmul(v, ph);
-          }
-        } else {
-          const uint32_t t = op.t;
-          for (uint32_t p = tid; p < (N >> 1); p += kThreads) {
-            const uint32_t l0 = insert_zero(p, t);
-            const uint32_t l1 = l0 ^ xl;
-            cx<T> v0 = tile[l1];
-            cx<T> v1 = tile[l0];
-            if ((__popc(l0 & zl) & 1u) ^ sgb) v0 = cneg(v0);
-            if ((__popc(l1 & zl) & 1u) ^ sgb) v1 = cneg(v1);
-            tile[l0] = cmul(v0, ph);
-            tile[l1] = cmul(v1, ph);
-          }
-        }
-        break;
-      }
-      case K_PERM: {
-        const uint32_t k = op.k;
-        const uint32_t D = 1u << k;
-        uint32_t tmask = 0;
-        for (uint32_t b = 0; b < k; ++b) tmask |= 1u << op.pos[b];
-        if (tid < D) {
-          uint32_t d = 0;
-          for (uint32_t b = 0; b < k; ++b) d |= ((tid >> b) & 1u) << op.pos[b];
-          doff[tid] = d;
-        }
-        __syncthreads();
-        // Groups: tile indices with every permuted bit clear.
-        for (uint32_t l = tid; l < N; l += kThreads) {
-          if (l & tmask) continue;
-          if ((l & cl) != cvl) continue;
-          for (uint32_t q = 0; q < op.npairs; ++q) {
-            const cx<double> pr = pool[op.off + q];
-            const uint32_t ia = l | doff[(uint32_t)pr.x];
-            const uint32_t ib = l | doff[(uint32_t)pr.y];
-            const cx<T> va = tile[ia];
-            tile[ia] = tile[ib];
-            tile[ib] = va;
-          }
-        }
-        break;
-      }
-      default:
-        break;
-    }
-  }
-  __syncthreads();
-  for (uint32_t l = tid; l < N; l += kThreads)
-    sv[base + (l & cmask) + hioff[l >> c]] = tile[l];
-}
-
-// ---------------------------------------------------------------------------
 // Dense matrix on 6..10 qubits: one workgroup per group of 2^k amplitudes.
 // The matrix (column-major) is read from global memory; consecutive threads
 // read consecutive rows of one column, so the reads coalesce.
@@ -289,6 +76,38 @@ __global__ void k_fill_basis(cx<T> *__restrict__ sv, uint64_t n, uint64_t one_at
   for (uint64_t i = blockIdx.x * (uint64_t)kThreads + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * kThreads)
     sv[i] = mk<T>(i == one_at ? T(1) : T(0), T(0));
+}
+
+// splitmix64 finaliser: a counter-based 64-bit hash.
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+// psi_i = (g1 + i g2)/sqrt(2) with g1, g2 standard normal (Box-Muller on two
+// 53-bit uniforms from the hash of (seed, i)); normalised afterwards.
+template <typename T>
+__global__ void k_gaussian(cx<T> *__restrict__ sv, uint64_t n, uint64_t seed) {
+  const uint64_t key = mix64(seed + 0x9e3779b97f4a7c15ull);
+  for (uint64_t i = blockIdx.x * (uint64_t)kThreads + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * kThreads) {
+    const uint64_t a = mix64(key ^ (2 * i)), b = mix64(key ^ (2 * i + 1));
+    const double u1 = ((a >> 11) + 1) * (1.0 / 9007199254740993.0);  // (0, 1]
+    const double u2 = (b >> 11) * (1.0 / 9007199254740992.0);        // [0, 1)
+    const double r = sqrt(-2.0 * log(u1)) * 0.70710678118654752;
+    double s, c;
+    sincospi(2.0 * u2, &s, &c);
+    sv[i] = mk<T>(static_cast<T>(r * c), static_cast<T>(r * s));
+  }
+}
+
+template <typename T>
+__global__ void k_scale(cx<T> *__restrict__ sv, uint64_t n, const double *__restrict__ sumsq) {
+  const double f = 1.0 / sqrt(*sumsq);
+  for (uint64_t i = blockIdx.x * (uint64_t)kThreads + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * kThreads)
+    sv[i] = mk<T>(static_cast<T>(sv[i].x * f), static_cast<T>(sv[i].y * f));
 }
 
 template <typename TO, typename TI>
@@ -476,6 +295,42 @@ __global__ void k_sample_linear(const cx<T> *__restrict__ sv, uint64_t size,
     if (rnd < p) break;
   }
   out[s] = sample;
+}
+
+struct PauliTerm {
+  uint64_t x, z;  // flip and sign masks
+  double cr, ci;  // coefficient times the Pauli phase (-i)^#Y
+};
+
+// out[i] = sum_t c_t (-1)^popcount(i & z_t) in[i ^ x_t] for i < half, where
+// in = sv[0, half) and out = sv[half, 2 half): the action of a Pauli sum,
+// (P psi)_i = phase (-1)^popcount(i & z) psi_{i ^ x}, as in
+// QubitVector::apply_pauli (qubitvector.hpp:2393-2433).  Terms arrive sorted
+// by x so each distinct flip is loaded once.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_pauli_sum_upper(cx<T> *__restrict__ sv,
+                                                              uint64_t half,
+                                                              const PauliTerm *__restrict__ terms,
+                                                              uint32_t nterms) {
+  for (uint64_t i = blockIdx.x * (uint64_t)kThreads + threadIdx.x; i < half;
+       i += (uint64_t)gridDim.x * kThreads) {
+    double ar = 0, ai = 0;
+    uint64_t xprev = ~0ull;
+    double vr = 0, vi = 0;
+    for (uint32_t t = 0; t < nterms; ++t) {
+      const PauliTerm pt = terms[t];
+      if (pt.x != xprev) {
+        const cx<T> v = sv[i ^ pt.x];
+        vr = v.x;
+        vi = v.y;
+        xprev = pt.x;
+      }
+      const double s = (__popcll(i & pt.z) & 1) ? -1.0 : 1.0;
+      ar += s * (pt.cr * vr - pt.ci * vi);
+      ai += s * (pt.cr * vi + pt.ci * vr);
+    }
+    sv[half + i] = mk<T>(static_cast<T>(ar), static_cast<T>(ai));
+  }
 }
 
 // <P> for one Pauli string (qubitvector.hpp:2300-2347), FP64 accumulation.

@@ -83,6 +83,10 @@ class QubitVector:
         p = {"double": L.PRECISION_DOUBLE, "single": L.PRECISION_SINGLE}[precision]
         L.check(self._L.aerhip_qv_initialize_from_device(self._h, C.c_void_p(ptr), length, p))
 
+    def initialize_random(self, seed: int) -> None:
+        """psi = g/|g| with i.i.d. complex Gaussian g generated on the device."""
+        L.check(self._L.aerhip_qv_initialize_random(self._h, int(seed)))
+
     def initialize_component(self, qubits, state) -> None:
         a, qp, n = _q(qubits)
         s = _c(state)
@@ -118,9 +122,12 @@ class QubitVector:
         return ms.value, n.value
 
     def stats(self) -> dict:
-        out = (C.c_uint64 * 3)()
+        out = (C.c_uint64 * 4)()
         L.check(self._L.aerhip_qv_stats(self._h, out))
-        return {"passes": out[0], "launches": out[1], "gates": out[2]}
+        return {"passes": out[0], "launches": out[1], "gates": out[2], "gathers": out[3]}
+
+    def set_register_bits(self, reg_bits: int) -> None:
+        L.check(self._L.aerhip_qv_set_register_bits(self._h, int(reg_bits)))
 
     # gates ------------------------------------------------------------------
     def apply_matrix(self, qubits, mat) -> None:
@@ -208,6 +215,28 @@ class QubitVector:
         out = np.empty(r.size, dtype=np.uint64)
         L.check(self._L.aerhip_sample_measure(self._h, r.ctypes.data, r.size, out.ctypes.data))
         return out
+
+    def save_expval_pauli(self, qubits, pauli: str, phase: complex = 1.0) -> int:
+        """Queues Re<psi|phase P|psi> at this point of the gate sequence; returns its slot."""
+        a, qp, n = _q(qubits)
+        phase = complex(phase)
+        slot = C.c_uint64()
+        L.check(self._L.aerhip_save_expval_pauli(self._h, qp, n, pauli.encode(), phase.real,
+                                                 phase.imag, C.byref(slot)))
+        return slot.value
+
+    def saved_expvals(self, capacity: int = 1 << 16) -> np.ndarray:
+        """Flushes and returns every saved value in slot order (then clears them)."""
+        out = np.empty(capacity, dtype=np.float64)
+        cnt = C.c_uint64()
+        L.check(self._L.aerhip_saved_expvals(self._h, out.ctypes.data, capacity, C.byref(cnt)))
+        return out[: cnt.value].copy()
+
+    def expval_pauli_multi(self, qubits, labels) -> np.ndarray:
+        """<P_t> for many Pauli strings on the same qubits, evaluated in one sweep."""
+        for lab in labels:
+            self.save_expval_pauli(qubits, lab)
+        return self.saved_expvals(max(1, len(labels)))
 
     def expval_pauli(self, qubits, pauli: str, phase: complex = 1.0) -> float:
         a, qp, n = _q(qubits)

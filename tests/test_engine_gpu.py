@@ -231,3 +231,47 @@ def test_errors_are_raised():
         e.initialize_from_vector(np.ones(7))
     with pytest.raises(L.AerhipInvalidArgument):
         e.apply_pauli([0], "Q")
+
+
+def test_random_state_is_normalised_gaussian():
+    print("case: initialize_random n=16", flush=True)
+    e = engine(16)
+    e.initialize_random(123)
+    v = e.copy_to_vector()
+    assert abs(np.linalg.norm(v) - 1) < 1e-12
+    z = v * np.sqrt(v.size)  # unit-variance complex Gaussian samples
+    assert abs(np.mean(np.abs(z) ** 2) - 1) < 0.02
+    assert abs(np.mean(z.real)) < 0.02 and abs(np.mean(z.imag)) < 0.02
+    e.initialize_random(123)
+    assert np.array_equal(e.copy_to_vector(), v)
+
+
+@pytest.mark.parametrize("precision,reg_bits", [("double", 0), ("double", 3), ("double", 4),
+                                                ("single", 0), ("single", 4), ("single", 5)])
+def test_register_layouts_match_oracle(precision, reg_bits):
+    """Every register-blocking setting gives the oracle's state (brickwork
+    U3+CX plus RY/RZ/CZ/SU(4)/swap layers on 16 qubits)."""
+    from qiskit_aer_amd import circuits
+    n = 16
+    print(f"case: register layouts {precision} R={reg_bits}", flush=True)
+    e, o, rng = pair(n, 40 + reg_bits, precision)
+    e.set_register_bits(reg_bits)
+    c = circuits.brickwork_u3_cx(n, 8, seed=5, measure=False)
+    e.apply_ops(c)
+    for op in c.ops:
+        if op.name == "u3":
+            o.apply_mcu(list(op.qubits), D.u3(*op.params))
+        else:
+            o.apply_mcx(list(op.qubits))
+    for q in range(n):
+        t = rng.uniform(-3, 3)
+        e.apply_rotation([q], "y", t); o.apply_rotation([q], "y", t)
+        e.apply_rotation([q], "z", -t); o.apply_rotation([q], "z", -t)
+    for a in range(0, n - 1, 2):
+        e.apply_mcphase([a, a + 1], -1); o.apply_mcphase([a, a + 1], -1)
+        m = D.random_unitary(4, rng)
+        e.apply_matrix([a + 1, a], m); o.apply_matrix([a + 1, a], m)
+    e.apply_mcswap([3, 12]); o.apply_mcswap([3, 12])
+    close(e, o, precision)
+    if reg_bits:
+        assert e.stats()["gathers"] > 0

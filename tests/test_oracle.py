@@ -43,6 +43,17 @@ def test_golden_statevectors(case):
 
 # ---- reference's own code (oracle/_ref) -----------------------------------
 
+def aligned64(a: np.ndarray) -> np.ndarray:
+    """Copy of `a` on a 64-byte boundary: the reference's AVX2 kernels use
+    aligned loads on the state, which it allocates with posix_memalign(64)
+    (qubitvector.hpp:951-967)."""
+    buf = np.empty(a.nbytes + 64, dtype=np.uint8)
+    off = (-buf.ctypes.data) % 64
+    out = buf[off:off + a.nbytes].view(a.dtype).reshape(a.shape)
+    out[...] = a
+    return out
+
+
 def ref_or_skip():
     R = O.ref()
     if R is None:
@@ -65,7 +76,7 @@ def test_apply_matrix_vs_reference_transformer(k, avx):
         v.initialize_from_vector(psi)
         v.apply_matrix(qubits, mat)
         mine = v.copy_to_vector()
-        theirs = psi.copy()
+        theirs = aligned64(psi)
         q = np.array(qubits, dtype=np.uint64)
         m = np.ascontiguousarray(mat.reshape(-1, order="F"))
         assert R.ref_apply_matrix(theirs.ctypes.data, n, q.ctypes.data_as(C.POINTER(C.c_uint64)), k,
@@ -87,7 +98,7 @@ def test_apply_diagonal_vs_reference_transformer(k):
     v.initialize_from_vector(psi)
     v.apply_diagonal_matrix(qubits, diag)
     for avx in (0, 1):
-        theirs = psi.copy()
+        theirs = aligned64(psi)
         q = np.array(qubits, dtype=np.uint64)
         assert R.ref_apply_diagonal_matrix(theirs.ctypes.data, n, q.ctypes.data_as(C.POINTER(C.c_uint64)),
                                            k, np.ascontiguousarray(diag).ctypes.data, avx) == 0
