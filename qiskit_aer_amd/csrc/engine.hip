@@ -9,6 +9,7 @@
 #include "engine.hpp"
 #include "gate_matrices.hpp"
 #include "kernels.hpp"
+#include "reg_pass.hpp"
 #include "tile_pass.hpp"
 
 namespace aerhip {
@@ -575,104 +576,397 @@ void Engine::initialize_component(const std::vector<uint32_t> &qubits,
 // and so does every later primitive that touches their qubits.
 // ---------------------------------------------------------------------------
 
-// Rewrites the LDS-form op list of a full tile pass into a register program:
-// 1-qubit matrices, 2-qubit matrices, X and Y gates whose mixed qubits are
-// the current register qubits run in registers; diagonals and phases run in
-// either form; everything else runs on the LDS tile.  A GATHER picks the
-// register qubits: the mixed qubits of the next gate plus those of the gates
-// that follow it, as long as they fit, then the highest free tile bits
-// (consecutive threads then read consecutive LDS words).
-std::vector<TileOp> register_program(const std::vector<TileOp> &in, uint32_t nb, uint32_t R,
-                                     uint64_t *gathers) {
-  // 0: runs in registers if its mixed qubits are register qubits, 1: runs in
-  // any form, 2: LDS only.
-  auto classify = [](const TileOp &op, uint32_t &tm) -> int {
-    switch (op.kind) {
-      case K_DENSE1:
-      case K_MCX:
-      case K_MCY:
-        tm = 1u << op.t;
-        return 0;
-      case K_DENSE:
-        if (op.k == 2) {
-          tm = (1u << op.pos[0]) | (1u << op.pos[1]);
-          return 0;
-        }
-        return 2;
-      case K_DIAG:
-      case K_MCPHASE:
-        tm = 0;
-        return 1;
-      default:
-        return 2;
-    }
-  };
-  auto regidx = [](uint32_t regset, uint32_t b) -> uint32_t {
-    return static_cast<uint32_t>(__builtin_popcount(regset & ((1u << b) - 1u)));
-  };
-  std::vector<TileOp> out;
-  out.reserve(in.size() * 2);
-  bool inreg = false;
-  uint32_t regset = 0;
-  TileOp scatter;
-  std::memset(&scatter, 0, sizeof(scatter));
-  scatter.kind = K_SCATTER;
-  for (size_t i = 0; i < in.size(); ++i) {
-    TileOp op = in[i];
-    uint32_t tm = 0;
-    const int cls = classify(op, tm);
-    if (cls == 0) {
-      if (!inreg || (tm & ~regset)) {
-        uint32_t rs = tm;
-        for (size_t j = i + 1; j < in.size(); ++j) {
-          uint32_t tj = 0;
-          const int cj = classify(in[j], tj);
-          if (cj == 1) continue;
-          if (cj == 2) break;
-          const uint32_t u = rs | tj;
-          if (static_cast<uint32_t>(__builtin_popcount(u)) > R) break;
-          rs = u;
-        }
-        for (int b = static_cast<int>(nb) - 1; b >= 0 && static_cast<uint32_t>(__builtin_popcount(rs)) < R; --b)
-          rs |= 1u << b;
-        if (inreg) out.push_back(scatter);
-        TileOp g;
-        std::memset(&g, 0, sizeof(g));
-        g.kind = K_GATHER;
-        uint32_t pr = 0, pt = R;
-        for (uint32_t b = 0; b < nb; ++b) {
-          if (rs & (1u << b)) g.pos[pr++] = static_cast<uint8_t>(b);
-          else g.pos[pt++] = static_cast<uint8_t>(b);
-        }
-        out.push_back(g);
-        if (gathers) ++*gathers;
-        regset = rs;
-        inreg = true;
+// Register program of a full tile pass (k_reg_pass).  Each TileOp is
+// classified: dense 2x2 / 4x4, X and Y and probes need their mixed tile bits
+// to be register bits; diagonals and phases run in any layout; everything
+// else runs on the LDS tile.  Ops keep program order except that an op may
+// move ahead of earlier ops it does not conflict with (disjoint tile bits, or
+// both diagonal, or both read-only).  The register set of each segment is the
+// R-subset of the tile bits needed by the next pending gates that lets the
+// most gates run before the next layout change.
+namespace {
+
+struct RItem {
+  TileOp op;
+  int cls;        // 0: register gate, 1: any layout, 2: LDS only
+  int cat;        // 0: general, 1: diagonal, 2: read-only
+  uint32_t need;  // tile bits that must be register bits
+  uint32_t touch; // tile bits read or written
+};
+
+RItem classify_item(const TileOp &op, uint32_t R) {
+  RItem it;
+  it.op = op;
+  it.cls = 2;
+  it.cat = 0;
+  it.need = 0;
+  it.touch = op.cl;
+  uint32_t pos = 0;
+  for (uint32_t j = 0; j < op.k && j < 16; ++j)
+    if (op.pos[j] != kNotLocal) pos |= 1u << op.pos[j];
+  switch (op.kind) {
+    case K_DENSE1:
+    case K_MCX:
+    case K_MCY:
+      it.cls = 0;
+      it.need = 1u << op.t;
+      it.touch |= it.need;
+      break;
+    case K_DENSE:
+      it.touch |= pos;
+      if (op.k == 2 && R >= 2) {
+        it.cls = 0;
+        it.need = pos;
       }
-      switch (op.kind) {
-        case K_DENSE1: op.kind = K_RDENSE1; op.t = regidx(regset, op.t); break;
-        case K_MCX: op.kind = K_RMCX; op.t = regidx(regset, op.t); break;
-        case K_MCY: op.kind = K_RMCY; op.t = regidx(regset, op.t); break;
-        default:
-          op.kind = K_RDENSE2;
-          op.pos[0] = static_cast<uint8_t>(regidx(regset, op.pos[0]));
-          op.pos[1] = static_cast<uint8_t>(regidx(regset, op.pos[1]));
-          break;
+      break;
+    case K_DIAG:
+      it.cls = 1;
+      it.cat = 1;
+      it.touch |= pos;
+      break;
+    case K_MCPHASE:
+      it.cls = 1;
+      it.cat = 1;
+      break;
+    case K_PROBE:
+      it.cat = 2;
+      it.touch = op.xl | op.zl;
+      if (static_cast<uint32_t>(__builtin_popcount(op.xl)) <= R) {
+        it.cls = 0;
+        it.need = op.xl;
       }
-      out.push_back(op);
-    } else if (cls == 1) {
-      if (inreg) op.kind = op.kind == K_DIAG ? K_RDIAG : K_RMCPHASE;
-      out.push_back(op);
+      break;
+    case K_PAULI:
+      it.touch |= op.xl | op.zl;
+      break;
+    default:
+      it.touch |= pos;
+      break;
+  }
+  return it;
+}
+
+// Scans `rem` in order; an item is taken if it conflicts with no earlier item
+// left behind and `ok(item)` holds.  Returns the taken indices.
+template <typename F>
+std::vector<size_t> scan_items(const std::vector<RItem> &rem, F ok) {
+  uint32_t blocked[3] = {0, 0, 0};
+  std::vector<size_t> taken;
+  for (size_t i = 0; i < rem.size(); ++i) {
+    const RItem &it = rem[i];
+    uint32_t conflict = blocked[0];
+    for (int c = 1; c < 3; ++c)
+      if (c != it.cat) conflict |= blocked[c];
+    if (!(it.touch & conflict) && ok(it)) {
+      taken.push_back(i);
     } else {
-      if (inreg) {
-        out.push_back(scatter);
-        inreg = false;
-      }
-      out.push_back(op);
+      blocked[it.cat] |= it.touch;
     }
   }
-  if (inreg) out.push_back(scatter);
-  return out;
+  return taken;
+}
+
+RLayout make_layout(uint32_t rs, uint32_t B, const uint8_t *spos) {
+  RLayout L;
+  std::memset(&L, 0, sizeof(L));
+  int r = 0, u = 0;
+  for (uint32_t b = 0; b < B; ++b) {
+    if (rs & (1u << b)) {
+      set_byte(L.rp, r, b);
+      set_byte(L.rq, r, spos[b]);
+      ++r;
+    } else {
+      set_byte(L.tp, u, b);
+      set_byte(L.tq, u, spos[b]);
+      ++u;
+    }
+  }
+  return L;
+}
+
+}  // namespace
+
+// Walks a register program the way k_reg_pass does and checks its shape and
+// every layout, so that a malformed program fails on the host and never
+// reaches the device.
+void validate_register_program(const ROp<double> *p, size_t n, uint32_t B, uint32_t R,
+                               const uint8_t *spos, size_t ntops) {
+  auto fail = [](const char *what) { throw std::logic_error(std::string("aerhip: register program: ") + what); };
+  auto check_layout = [&](const RLayout &L) {
+    uint32_t seen = 0;
+    for (uint32_t u = 0; u < B - R; ++u) {
+      const uint32_t b = byte_at(L.tp, static_cast<int>(u));
+      if (b >= B || byte_at(L.tq, static_cast<int>(u)) != spos[b]) fail("thread bit");
+      seen |= 1u << b;
+    }
+    for (uint32_t r = 0; r < R; ++r) {
+      const uint32_t b = byte_at(L.rp, static_cast<int>(r));
+      if (b >= B || byte_at(L.rq, static_cast<int>(r)) != spos[b]) fail("register bit");
+      seen |= 1u << b;
+    }
+    if (seen != (1u << B) - 1u) fail("layout is not a permutation");
+  };
+  size_t o = 0;
+  for (;;) {
+    if (o >= n) fail("missing phase");
+    const ROp<double> &e = p[o++];
+    if (e.kind != Q_LOAD && e.kind != Q_FILL) fail("phase must start with LOAD or FILL");
+    if ((o == 1) != (e.kind == Q_LOAD)) fail("LOAD must come first, and only there");
+    check_layout(e.u.lay);
+    if (e.extra & 1u) fail("odd phase length");
+    if (o + e.extra >= n) fail("phase overruns the program");
+    for (uint32_t i = 0; i < e.extra; ++i) {
+      const ROp<double> &r = p[o++];
+      switch (r.kind) {
+        case Q_RELAYOUT: check_layout(r.u.lay); break;
+        case Q_U1: case Q_U1C: case Q_X: case Q_Y:
+          if (r.a >= R) fail("register bit of a gate");
+          break;
+        case Q_U2:
+          if (r.a >= R || r.b >= R || r.a == r.b) fail("register bits of a 2-qubit gate");
+          break;
+        case Q_D1: if (r.a >= B) fail("diagonal bit"); break;
+        case Q_DIAG: if (r.extra >= ntops) fail("diagonal index"); break;
+        case Q_PROBE: if (r.u.pr.xr >= (1u << R)) fail("probe mask"); break;
+        case Q_PHASE: case Q_NOP: break;
+        default: fail("op kind inside a register phase");
+      }
+    }
+    const ROp<double> &l = p[o++];
+    if (l.kind == Q_STORE) {
+      check_layout(l.u.lay);
+      if (o != n) fail("ops after STORE");
+      return;
+    }
+    if (l.kind != Q_SPILL) fail("phase must end with STORE or SPILL");
+    while (o < n && p[o].kind == Q_LDS) {
+      if (p[o].extra >= ntops) fail("LDS op index");
+      ++o;
+    }
+  }
+}
+
+// Builds the register program of one pass.  ROps go to `rops` (matrices in
+// double; converted for complex64 at upload), TileOps run on the LDS tile go to
+// `tops`.  `lowb`: tile bits that must be lane bits for a full 128-byte run per
+// HBM access.
+void register_program(const std::vector<TileOp> &in, uint32_t B, uint32_t R, uint32_t lowb,
+                      const uint8_t *spos, const std::vector<cd> &pool,
+                      std::vector<ROp<double>> &rops, std::vector<TileOp> &tops,
+                      uint64_t *relayouts) {
+  std::vector<RItem> rem;
+  rem.reserve(in.size());
+  for (const TileOp &op : in) rem.push_back(classify_item(op, R));
+  const uint32_t lowmask = (1u << lowb) - 1u;
+  auto fits = [](const RItem &it, uint32_t rs) { return it.cls == 1 || (it.cls == 0 && (it.need & ~rs) == 0); };
+  auto score = [&](uint32_t rs) {
+    size_t n = 0;
+    for (size_t i : scan_items(rem, [&](const RItem &it) { return fits(it, rs); }))
+      if (rem[i].cls == 0) ++n;
+    return n;
+  };
+  // Fills rs up to R bits: bits that keep the low bits on lanes first, highest first.
+  auto fill = [&](uint32_t rs) {
+    for (int b = static_cast<int>(B) - 1; b >= 0 && static_cast<uint32_t>(__builtin_popcount(rs)) < R; --b)
+      if (!(lowmask & (1u << b))) rs |= 1u << b;
+    for (int b = static_cast<int>(B) - 1; b >= 0 && static_cast<uint32_t>(__builtin_popcount(rs)) < R; --b)
+      rs |= 1u << b;
+    return rs;
+  };
+  // Best register set for the pending gates; `compat` restricts to sets that
+  // leave the low bits on lanes.  Returns {set, score}.
+  auto best_set = [&](bool compat) {
+    uint32_t cand_bits = 0;
+    size_t seen = 0;
+    for (size_t i : scan_items(rem, [&](const RItem &it) { return it.cls != 2; })) {
+      if (rem[i].cls != 0) continue;
+      const uint32_t nb = cand_bits | rem[i].need;
+      if (static_cast<uint32_t>(__builtin_popcount(nb)) > R + 4) break;
+      cand_bits = nb;
+      if (++seen >= 24) break;
+    }
+    if (compat) cand_bits &= ~lowmask;
+    uint32_t best = fill(0);
+    size_t best_score = 0;
+    int best_low = 99;
+    // subsets of cand_bits with at most R bits, completed by fill()
+    std::vector<uint32_t> bits;
+    for (uint32_t b = 0; b < B; ++b)
+      if (cand_bits & (1u << b)) bits.push_back(b);
+    const uint32_t nbits = static_cast<uint32_t>(bits.size());
+    for (uint32_t m = 1; m < (1u << nbits); ++m) {
+      if (static_cast<uint32_t>(__builtin_popcount(m)) > R) continue;
+      if (static_cast<uint32_t>(__builtin_popcount(m)) < std::min<uint32_t>(R, nbits)) continue;
+      uint32_t rs = 0;
+      for (uint32_t i = 0; i < nbits; ++i)
+        if (m & (1u << i)) rs |= 1u << bits[i];
+      rs = fill(rs);
+      if (compat && (rs & lowmask)) continue;
+      const size_t sc = score(rs);
+      const int low = __builtin_popcount(rs & lowmask);
+      if (sc > best_score || (sc == best_score && sc > 0 && low < best_low)) {
+        best = rs;
+        best_score = sc;
+        best_low = low;
+      }
+    }
+    return std::make_pair(best, best_score);
+  };
+  // A register phase is ENTER (LOAD/FILL, whose `extra` counts the register
+  // ops that follow), an even number of register ops (Q_NOP pads), then
+  // LEAVE (STORE/SPILL).
+  size_t enter_at = 0;
+  auto emit = [&](uint32_t kind, uint32_t rs) {
+    if (kind == Q_STORE || kind == Q_SPILL) {
+      uint32_t cnt = static_cast<uint32_t>(rops.size() - enter_at - 1);
+      if (cnt & 1u) {
+        ROp<double> nop;
+        std::memset(&nop, 0, sizeof(nop));
+        nop.kind = Q_NOP;
+        rops.push_back(nop);
+        ++cnt;
+      }
+      rops[enter_at].extra = cnt;
+    }
+    if (kind == Q_LOAD || kind == Q_FILL) enter_at = rops.size();
+    ROp<double> r;
+    std::memset(&r, 0, sizeof(r));
+    r.kind = kind;
+    r.u.lay = make_layout(rs, B, spos);
+    rops.push_back(r);
+  };
+  auto regidx = [](uint32_t rs, uint32_t b) {
+    return static_cast<uint32_t>(__builtin_popcount(rs & ((1u << b) - 1u)));
+  };
+  auto emit_gate = [&](const TileOp &op, uint32_t rs) {
+    ROp<double> r;
+    std::memset(&r, 0, sizeof(r));
+    r.cl = op.cl;
+    r.cvl = op.cvl;
+    r.cg = op.cg;
+    r.cvg = op.cvg;
+    switch (op.kind) {
+      case K_DENSE1:
+        r.kind = (op.cl || op.cg) ? Q_U1C : Q_U1;
+        r.a = regidx(rs, op.t);
+        for (int e = 0; e < 4; ++e) {
+          r.u.m[2 * e] = pool[op.off + e].real();
+          r.u.m[2 * e + 1] = pool[op.off + e].imag();
+        }
+        break;
+      case K_MCX:
+      case K_MCY:
+        r.kind = op.kind == K_MCX ? Q_X : Q_Y;
+        r.a = regidx(rs, op.t);
+        break;
+      case K_DENSE:
+        r.kind = Q_U2;
+        r.a = regidx(rs, op.pos[0]);
+        r.b = regidx(rs, op.pos[1]);
+        r.extra = op.off;
+        break;
+      case K_DIAG:
+        if (op.k == 1 && op.pos[0] != kNotLocal) {
+          r.kind = Q_D1;
+          r.a = op.pos[0];
+          for (int e = 0; e < 2; ++e) {
+            r.u.m[2 * e] = pool[op.off + e].real();
+            r.u.m[2 * e + 1] = pool[op.off + e].imag();
+          }
+        } else {
+          r.kind = Q_DIAG;
+          r.extra = static_cast<uint32_t>(tops.size());
+          tops.push_back(op);
+        }
+        break;
+      case K_MCPHASE:
+        r.kind = Q_PHASE;
+        r.u.m[0] = op.pr;
+        r.u.m[1] = op.pi;
+        break;
+      case K_PROBE: {
+        r.kind = Q_PROBE;
+        r.extra = op.npairs;
+        uint32_t xr = 0;
+        for (uint32_t b = 0; b < B; ++b)
+          if (op.xl & (1u << b)) xr |= 1u << regidx(rs, b);
+        r.u.pr.xr = xr;
+        r.u.pr.zl = op.zl;
+        r.u.pr.zg = op.zg;
+        r.u.pr.pr = op.pr;
+        r.u.pr.pi = op.pi;
+        break;
+      }
+      default:
+        throw std::logic_error("aerhip: op cannot run in registers");
+    }
+    rops.push_back(r);
+  };
+  auto emit_lds = [&](const TileOp &op) {
+    ROp<double> r;
+    std::memset(&r, 0, sizeof(r));
+    r.kind = Q_LDS;
+    r.extra = static_cast<uint32_t>(tops.size());
+    tops.push_back(op);
+    rops.push_back(r);
+  };
+  auto take = [&](const std::vector<size_t> &idx, uint32_t rs, bool lds_mode) {
+    for (size_t i : idx) {
+      if (lds_mode) emit_lds(rem[i].op);
+      else emit_gate(rem[i].op, rs);
+    }
+    std::vector<RItem> keep;
+    keep.reserve(rem.size() - idx.size());
+    size_t k = 0;
+    for (size_t i = 0; i < rem.size(); ++i) {
+      if (k < idx.size() && idx[k] == i) {
+        ++k;
+        continue;
+      }
+      keep.push_back(rem[i]);
+    }
+    rem.swap(keep);
+  };
+
+  // Load in the best set that keeps the low bits on lanes.
+  uint32_t rs = best_set(true).first;
+  emit(Q_LOAD, rs);
+  bool lds_mode = false;
+  while (!rem.empty()) {
+    if (!lds_mode) {
+      const auto idx = scan_items(rem, [&](const RItem &it) { return fits(it, rs); });
+      if (!idx.empty()) {
+        take(idx, rs, false);
+        continue;
+      }
+      const auto b = best_set(false);
+      if (b.second > 0) {
+        rs = b.first;
+        emit(Q_RELAYOUT, rs);
+        if (relayouts) ++*relayouts;
+        continue;
+      }
+      emit(Q_SPILL, rs);
+      lds_mode = true;
+    }
+    // LDS mode: run every ready op that cannot run in registers (and ready
+    // diagonals), then return to registers if register gates are pending.
+    const auto idx = scan_items(rem, [&](const RItem &it) { return it.cls != 0; });
+    if (!idx.empty()) {
+      take(idx, rs, true);
+      continue;
+    }
+    rs = best_set(false).first;
+    emit(Q_FILL, rs);
+    if (relayouts) ++*relayouts;
+    lds_mode = false;
+  }
+  if (lds_mode || (rs & lowmask)) {
+    rs = fill(0);
+    emit(lds_mode ? Q_FILL : Q_RELAYOUT, rs);
+    if (relayouts) ++*relayouts;
+  }
+  emit(Q_STORE, rs);
 }
 
 std::vector<PassPlan> Engine::plan() const {
@@ -734,6 +1028,7 @@ void Engine::flush() {
 
   std::vector<PassDesc> passes;
   std::vector<TileOp> ops;
+  std::vector<ROp<double>> rops;
   passes.reserve(plans.size());
   for (const PassPlan &pp : plans) {
     PassDesc pd;
@@ -810,19 +1105,25 @@ void Engine::flush() {
     const uint32_t full_bits =
         prec_ == AERHIP_PRECISION_DOUBLE ? kTileBitsC128 : kTileBitsC64;
     if (!pp.solo && reg_bits_ > 0 && ns == full_bits) {
-      pd.kind = 2;  // register-blocked tile pass
-      pops = register_program(pops, ns, reg_bits_, &stats_.gathers);
+      pd.kind = 2;  // register-layout tile pass: op_begin/nops index the ROps
+      pd.op_begin = static_cast<uint32_t>(rops.size());
+      register_program(pops, ns, reg_bits_, low_bits_, pd.spos, pool_, rops, ops, &stats_.gathers);
+      pd.nops = static_cast<uint32_t>(rops.size()) - pd.op_begin;
+      validate_register_program(rops.data() + pd.op_begin, pd.nops, ns, reg_bits_, pd.spos, ops.size());
+    } else {
+      pd.nops = static_cast<uint32_t>(pops.size());
+      ops.insert(ops.end(), pops.begin(), pops.end());
     }
-    pd.nops = static_cast<uint32_t>(pops.size());
-    ops.insert(ops.end(), pops.begin(), pops.end());
     passes.push_back(pd);
   }
 
-  // Upload [passes | ops | pool] with one copy from pinned staging.
+  // Upload [passes | ops | rops | pool] with one copy from pinned staging.
   const size_t pass_bytes = passes.size() * sizeof(PassDesc);
   const size_t ops_off = align_up(pass_bytes, 256);
   const size_t ops_bytes = ops.size() * sizeof(TileOp);
-  const size_t pool_off = align_up(ops_off + ops_bytes, 256);
+  const size_t rops_off = align_up(ops_off + ops_bytes, 256);
+  const size_t rops_bytes = rops.size() * sizeof(ROp<double>);
+  const size_t pool_off = align_up(rops_off + rops_bytes, 256);
   const size_t total = pool_off + pool_.size() * sizeof(cd) + 16;
   AERHIP_CHECK(hipEventSynchronize(prog_ev_));  // previous upload finished with staging
   if (total > h_prog_cap_) {
@@ -839,6 +1140,18 @@ void Engine::flush() {
   char *h = static_cast<char *>(h_prog_);
   std::memcpy(h, passes.data(), pass_bytes);
   std::memcpy(h + ops_off, ops.data(), ops_bytes);
+  if (prec_ == AERHIP_PRECISION_DOUBLE) {
+    std::memcpy(h + rops_off, rops.data(), rops_bytes);
+  } else {
+    // complex64 programs carry their matrices in float
+    auto *rf = reinterpret_cast<ROp<float> *>(h + rops_off);
+    for (size_t i = 0; i < rops.size(); ++i) {
+      std::memcpy(&rf[i], &rops[i], sizeof(ROp<float>));
+      const uint32_t k = rops[i].kind;
+      if (k == Q_U1 || k == Q_U1C || k == Q_D1 || k == Q_PHASE)
+        for (int e = 0; e < 8; ++e) rf[i].u.m[e] = static_cast<float>(rops[i].u.m[e]);
+    }
+  }
   if (!pool_.empty()) std::memcpy(h + pool_off, pool_.data(), pool_.size() * sizeof(cd));
   AERHIP_CHECK(hipMemcpyAsync(d_prog_, h_prog_, total, hipMemcpyHostToDevice, stream_));
   AERHIP_CHECK(hipEventRecord(prog_ev_, stream_));
@@ -846,6 +1159,7 @@ void Engine::flush() {
   char *d = static_cast<char *>(d_prog_);
   const PassDesc *dpass = reinterpret_cast<const PassDesc *>(d);
   const TileOp *dops = reinterpret_cast<const TileOp *>(d + ops_off);
+  const void *drops = d + rops_off;
   const cx<double> *dpool = reinterpret_cast<const cx<double> *>(d + pool_off);
   // Probe partials: one double per (probe, tile); every tile pass has
   // 2^(n - tile_bits) tiles.
@@ -869,20 +1183,22 @@ void Engine::flush() {
       const uint32_t grid = static_cast<uint32_t>(1ull << pd.ncomp);
       if (prec_ == AERHIP_PRECISION_DOUBLE) {
         auto *sv = static_cast<cx<double> *>(sv_);
+        auto *ro = static_cast<const ROp<double> *>(drops);
         if (reg_bits_ == 3)
-          k_tile_reg<double, kTileBitsC128, 3><<<grid, 1 << (kTileBitsC128 - 3), 0, stream_>>>(
-              sv, dpass + i, dops, dpool, dprobes);
+          k_reg_pass<double, kTileBitsC128, 3><<<grid, 1 << (kTileBitsC128 - 3), 0, stream_>>>(
+              sv, dpass + i, ro, dops, dpool, dprobes);
         else
-          k_tile_reg<double, kTileBitsC128, 4><<<grid, 1 << (kTileBitsC128 - 4), 0, stream_>>>(
-              sv, dpass + i, dops, dpool, dprobes);
+          k_reg_pass<double, kTileBitsC128, 4><<<grid, 1 << (kTileBitsC128 - 4), 0, stream_>>>(
+              sv, dpass + i, ro, dops, dpool, dprobes);
       } else {
         auto *sv = static_cast<cx<float> *>(sv_);
+        auto *ro = static_cast<const ROp<float> *>(drops);
         if (reg_bits_ == 4)
-          k_tile_reg<float, kTileBitsC64, 4><<<grid, 1 << (kTileBitsC64 - 4), 0, stream_>>>(
-              sv, dpass + i, dops, dpool, dprobes);
+          k_reg_pass<float, kTileBitsC64, 4><<<grid, 1 << (kTileBitsC64 - 4), 0, stream_>>>(
+              sv, dpass + i, ro, dops, dpool, dprobes);
         else
-          k_tile_reg<float, kTileBitsC64, 5><<<grid, 1 << (kTileBitsC64 - 5), 0, stream_>>>(
-              sv, dpass + i, dops, dpool, dprobes);
+          k_reg_pass<float, kTileBitsC64, 5><<<grid, 1 << (kTileBitsC64 - 5), 0, stream_>>>(
+              sv, dpass + i, ro, dops, dpool, dprobes);
       }
     } else if (pd.kind == 0) {
       // LDS-resident tile pass: the same kernel without register layouts.

@@ -194,7 +194,8 @@ __device__ __forceinline__ double block_sum_nt(double v, double *red) {
 // LDS-resident primitive (the tile is in `tile`); every thread takes part.
 // K_PROBE writes the tile's share of a Pauli expectation value to
 // probes[slot * ntiles + tile].
-template <typename T, int NT, int BMAX>
+// MATCAP: complex entries of `mat`; larger matrices are read from the pool.
+template <typename T, int NT, int BMAX, int MATCAP = 1024>
 __device__ __forceinline__ void lds_apply(const TileOp &op, cx<T> *tile, cx<T> *mat,
                                           uint32_t *doff, double *red, uint32_t N, uint64_t base,
                                           const cx<double> *__restrict__ pool,
@@ -252,7 +253,9 @@ __device__ __forceinline__ void lds_apply(const TileOp &op, cx<T> *tile, cx<T> *
       const uint32_t k = op.k, D = 1u << k;
       uint32_t tmask = 0;
       for (uint32_t b = 0; b < k; ++b) tmask |= 1u << op.pos[b];
-      for (uint32_t e = tid; e < D * D; e += NT) mat[e] = to_t<T>(pool[op.off + e]);
+      const bool staged = D * D <= static_cast<uint32_t>(MATCAP);
+      if (staged)
+        for (uint32_t e = tid; e < D * D; e += NT) mat[e] = to_t<T>(pool[op.off + e]);
       if (tid < D) {
         uint32_t d = 0;
         for (uint32_t b = 0; b < k; ++b) d |= ((tid >> b) & 1u) << op.pos[b];
@@ -271,7 +274,11 @@ __device__ __forceinline__ void lds_apply(const TileOp &op, cx<T> *tile, cx<T> *
             for (uint32_t b = 0; b < k; ++b) i |= ((l >> op.pos[b]) & 1u) << b;
             const uint32_t lb = l & ~tmask;
             cx<T> acc = mk<T>(0, 0);
-            for (uint32_t j = 0; j < D; ++j) acc = cfma(mat[i + D * j], tile[swz<T>(lb | doff[j])], acc);
+            if (staged)
+              for (uint32_t j = 0; j < D; ++j) acc = cfma(mat[i + D * j], tile[swz<T>(lb | doff[j])], acc);
+            else
+              for (uint32_t j = 0; j < D; ++j)
+                acc = cfma(to_t<T>(pool[op.off + i + D * j]), tile[swz<T>(lb | doff[j])], acc);
             out[r] = acc;
           }
         }

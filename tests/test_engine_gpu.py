@@ -272,6 +272,24 @@ def test_register_layouts_match_oracle(precision, reg_bits):
         m = D.random_unitary(4, rng)
         e.apply_matrix([a + 1, a], m); o.apply_matrix([a + 1, a], m)
     e.apply_mcswap([3, 12]); o.apply_mcswap([3, 12])
+    # controlled gates with controls inside and outside the tile, Y, and a
+    # 2-qubit diagonal
+    for ctrl, tgt in ((9, 1), (15, 2), (0, 14), (13, 7)):
+        u = D.random_unitary(2, rng)
+        e.apply_mcu([ctrl, tgt], u); o.apply_mcu([ctrl, tgt], u)
+    e.apply_mcx([2, 15, 5]); o.apply_mcx([2, 15, 5])
+    e.apply_mcy([6, 1]); o.apply_mcy([6, 1])
+    d = np.exp(1j * rng.uniform(-3, 3, 4))
+    e.apply_diagonal_matrix([4, 11], d); o.apply_diagonal_matrix([4, 11], d)
+    for q in (0, 5, 10, 15):
+        m = D.random_unitary(2, rng)
+        e.apply_matrix([q], m); o.apply_matrix([q], m)
+    labels = ["XYZIX", "ZZIIY", "IXXYZ"]
+    for lab in labels:
+        e.save_expval_pauli([1, 4, 9, 12, 15], lab)
+    got = e.saved_expvals()
+    want = [o.expval_pauli([1, 4, 9, 12, 15], lab) for lab in labels]
+    assert np.max(np.abs(np.array(got) - np.array(want))) < (1e-12 if precision == "double" else 1e-5)
     close(e, o, precision)
     if reg_bits:
         assert e.stats()["gathers"] > 0
