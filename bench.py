@@ -127,6 +127,75 @@ def run_adjoint(n: int, layers: int, batch: int, shift_params: int, seed: int) -
             "max_abs_adjoint_minus_shift": float(np.max(np.abs(grad[0, which] - ps[0])))}
 
 
+def run_mixed_precision(n: int, layers: int, seed: int) -> dict:
+    """Haar-SU(4) brickwork on n qubits in complex128 and complex64, both
+    states resident on the device: gate throughput of each, 1 - |<psi64|psi128>|^2
+    and the mean |error| of <Z_i Z_{i+1}>."""
+    from qiskit_aer_amd import QubitVector, circuits
+    c = circuits.su4_brickwork(n, layers, seed)
+    packed = c.pack()
+    out = {"gates": c.count_gates(), "num_qubits": n, "layers": layers}
+    states = {}
+    labels = []
+    for i in range(n - 1):
+        lab = ["I"] * n
+        lab[n - 1 - i] = lab[n - 2 - i] = "Z"
+        labels.append("".join(lab))
+    zz = {}
+    for prec in ("double", "single"):
+        qv = QubitVector(n, prec)
+        qv.initialize()
+        qv.synchronize()
+        qv.set_event_timing(True)
+        t0 = time.perf_counter()
+        qv.apply_ops(packed)
+        qv.synchronize()
+        wall = time.perf_counter() - t0
+        ms, launches = qv.pass_timing()
+        out[prec] = {"gates_per_s": c.count_gates() / wall, "ms": 1e3 * wall,
+                     "kernel_ms": ms, "passes": launches}
+        zz[prec] = qv.expval_pauli_multi(list(range(n)), labels)
+        states[prec] = qv
+    ov = states["single"].inner_product(states["double"])
+    out["one_minus_fidelity"] = 1.0 - abs(ov) ** 2
+    out["mean_abs_zz_error"] = float(np.mean(np.abs(zz["single"] - zz["double"])))
+    out["c64_over_c128"] = out["single"]["gates_per_s"] / out["double"]["gates_per_s"]
+    for qv in states.values():
+        qv.close()
+    return out
+
+
+def run_batched(num_circuits: int, terms: int, seed: int) -> dict:
+    """num_circuits hardware-efficient ansatz circuits, n ~ U{10..20} qubits,
+    reps ~ U{2..8}, each with a `terms`-term random Pauli sum: circuits/s for one
+    batched launch (complex128 and complex64) and circuit by circuit."""
+    from qiskit_aer_amd import circuits
+    from qiskit_aer_amd.batch import batch_pauli_expval
+    rng = np.random.default_rng(seed)
+    cs, hs = [], []
+    for i in range(num_circuits):
+        n = int(rng.integers(10, 21))
+        c, _ = circuits.hea(n, int(rng.integers(2, 9)), seed + i)
+        cs.append(c)
+        hs.append(circuits.random_pauli_sum(n, terms, seed + 7919 + i))
+    out = {"circuits": num_circuits, "terms": terms, "gates": sum(c.count_gates() for c in cs)}
+    energies = {}
+    for prec, mode in (("double", "batch"), ("single", "batch"), ("double", "sequential")):
+        t0 = time.perf_counter()
+        e, ms = batch_pauli_expval(cs, hs, prec, mode)
+        wall = time.perf_counter() - t0
+        energies[(prec, mode)] = e
+        out[f"{mode}_{'c128' if prec == 'double' else 'c64'}"] = {
+            "circuits_per_s": num_circuits / wall, "wall_s": wall, "kernel_ms": ms}
+    out["max_abs_energy_diff_batch_vs_sequential"] = float(
+        np.max(np.abs(energies[("double", "batch")] - energies[("double", "sequential")])))
+    out["max_abs_energy_diff_c64_vs_c128"] = float(
+        np.max(np.abs(energies[("single", "batch")] - energies[("double", "batch")])))
+    out["batch_over_sequential"] = (out["batch_c128"]["circuits_per_s"]
+                                    / out["sequential_c128"]["circuits_per_s"])
+    return out
+
+
 def cpu_baseline(n: int, layers: int, seed: int) -> dict:
     """The oracle (restated reference CPU path, fusion on, OpenMP on every
     host core) on the first `layers` layers of the same brickwork."""
@@ -183,6 +252,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--adjoint-batch", type=int, default=64)
+    ap.add_argument("--mixed-qubits", type=int, default=32)
+    ap.add_argument("--batch-circuits", type=int, default=2048)
+    ap.add_argument("--big-qubits", type=int, default=33, help="0 skips the large-state leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -217,6 +289,23 @@ def main() -> int:
         log(f"bench: adjoint 24q {adj['grads_per_s']:.0f} grads/s, param shift "
             f"{adj['param_shift_grads_per_s']:.1f} grads/s ({adj['speedup_vs_param_shift']:.0f}x)")
         extra["adjoint_24q_c128"] = adj
+        mp = run_mixed_precision(args.mixed_qubits, 20, seed)
+        log(f"bench: su4 {args.mixed_qubits}q c128 {mp['double']['gates_per_s']:.0f} gates/s, c64 "
+            f"{mp['single']['gates_per_s']:.0f} gates/s, 1-F {mp['one_minus_fidelity']:.2e}, "
+            f"|dZZ| {mp['mean_abs_zz_error']:.2e}")
+        extra[f"su4_brickwork_{args.mixed_qubits}q"] = mp
+        if args.big_qubits:
+            big = circuits.brickwork_u3_cx(args.big_qubits, args.layers, seed, measure=False)
+            br = run_engine(args.big_qubits, "double", big, 1, 0)
+            log(f"bench: brickwork {args.big_qubits}q c128 {br['gates_per_s']:.1f} gates/s, "
+                f"kernel {br['kernel_gbs']:.0f} GB/s")
+            extra[f"brickwork_{args.big_qubits}q_c128"] = {
+                k: br[k] for k in ("gates_per_s", "ms_per_step", "hbm_gbs", "kernel_gbs", "passes_per_step")}
+        bt = run_batched(args.batch_circuits, 64, seed)
+        log(f"bench: batch {bt['circuits']} circuits: one launch {bt['batch_c128']['circuits_per_s']:.0f} "
+            f"circuits/s (c64 {bt['batch_c64']['circuits_per_s']:.0f}), circuit by circuit "
+            f"{bt['sequential_c128']['circuits_per_s']:.0f} circuits/s")
+        extra["batched_hea_circuits"] = bt
 
     cpu = None
     if not args.no_cpu:

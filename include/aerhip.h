@@ -205,6 +205,12 @@ int aerhip_apply_multi_swaps(aerhip_qv *qv, const uint64_t *qubits,
 /* ---- readout (flush the queue; FP64 reductions) ------------------------- */
 
 /* QubitVector::norm() (qubitvector.hpp:1879-1886). */
+/* <a|b> = sum_i conj(a_i) b_i, accumulated in FP64 (the states may differ in
+ * precision; same qubit count and device).  No counterpart in the reference's
+ * QubitVector; it serves the complex64-vs-complex128 fidelity check
+ * 1 - |<psi64|psi128>|^2 on device-resident states. */
+int aerhip_qv_inner_product(aerhip_qv *a, aerhip_qv *b, double *re, double *im);
+
 int aerhip_norm(aerhip_qv *qv, double *out);
 
 /* QubitVector::probability(outcome) (qubitvector.hpp:2090-2092). */
@@ -316,6 +322,31 @@ int aerhip_run_circuit(const aerhip_op *ops, uint64_t num_ops,
                        uint64_t num_qubits, uint64_t num_memory,
                        const aerhip_run_config *cfg, uint64_t *memory_out,
                        double *statevector_out, double *timing_out);
+
+/* ---- batched circuits ----------------------------------------------------- */
+
+/* Runs `num_circuits` independent circuits, each from |0> on its own
+ * num_qubits[i] qubits, and returns energies[i] = sum_t coeffs[t] <P_t> over
+ * the circuit's Pauli terms.  Replaces running each circuit as a separate
+ * experiment (the reference's controller loop over circuits,
+ * src/controllers/aer_controller.hpp:309-437, with EstimatorV2's expectation
+ * values, qiskit_aer/primitives/estimator_v2.py:112-159).
+ *   ops          all circuits' unitary operations, concatenated; circuit i owns
+ *                ops[op_offsets[i] .. op_offsets[i+1]);
+ *   labels       all Pauli strings, concatenated: term t of circuit i has
+ *                num_qubits[i] characters (character n-1-q acts on qubit q);
+ *   coeffs       one real weight per term; circuit i owns terms
+ *                [term_offsets[i], term_offsets[i+1]);
+ *   mode         0: every circuit in one kernel launch (one workgroup walks all
+ *                passes of its circuit); 1: circuit by circuit, one launch per
+ *                pass (the reference's execution order), for comparison;
+ *   kernel_ms    (optional) device time of the execution.
+ * Dense matrices must act on at most 5 qubits. */
+int aerhip_batch_pauli_expval(const aerhip_op *ops, const uint64_t *op_offsets,
+                              const uint32_t *num_qubits, uint64_t num_circuits,
+                              const char *labels, const double *coeffs,
+                              const uint64_t *term_offsets, int precision, int device,
+                              int mode, double *energies, double *kernel_ms);
 
 /* ---- adjoint differentiation -------------------------------------------- */
 

@@ -106,6 +106,7 @@ _SIGNATURES = {
     "aerhip_apply_multiplexer": (C.c_int, [C.c_void_p, _u64p, C.c_uint64, _u64p, C.c_uint64, C.c_void_p]),
     "aerhip_apply_multi_swaps": (C.c_int, [C.c_void_p, _u64p, C.c_uint64]),
     "aerhip_norm": (C.c_int, [C.c_void_p, _dp]),
+    "aerhip_qv_inner_product": (C.c_int, [C.c_void_p, C.c_void_p, _dp, _dp]),
     "aerhip_probability": (C.c_int, [C.c_void_p, C.c_uint64, _dp]),
     "aerhip_probabilities": (C.c_int, [C.c_void_p, _u64p, C.c_uint64, C.c_void_p]),
     "aerhip_sample_measure": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
@@ -114,6 +115,9 @@ _SIGNATURES = {
                                            C.c_double, _u64p]),
     "aerhip_saved_expvals": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, _u64p]),
     "aerhip_qv_pauli_sum_to_upper_half": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p, C.c_uint64]),
+    "aerhip_batch_pauli_expval": (C.c_int, [C.POINTER(AerhipOp), _u64p, C.POINTER(C.c_uint32),
+                                            C.c_uint64, C.c_char_p, _dp, _u64p, C.c_int, C.c_int,
+                                            C.c_int, _dp, _dp]),
     "aerhip_adjoint_expval_gradient": (C.c_int, [C.POINTER(AerhipOp), C.c_uint64, C.c_uint64,
                                                  C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p,
                                                  C.c_char_p, C.c_void_p, C.c_uint64, C.c_int,

@@ -20,7 +20,11 @@ OBJ = ROOT / "_build"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("AERHIP_ARCH", "gfx950")
 
-# Packed-FP32 instructions are disabled for all device code.
+# Packed-FP32 instructions are disabled for all device code.  MFMA results
+# are kept in ordinary VGPRs (no AGPR copies, which would halve occupancy).
+# SimplifyCFG's sinking of common code is off: it merges the per-register-bit
+# gate variants into one path with computed register offsets, which forces
+# the register-resident amplitudes into scratch memory.
 HIP_FLAGS = [
     f"--offload-arch={ARCH}",
     "-O3",
@@ -29,6 +33,8 @@ HIP_FLAGS = [
     "-Wall",
     "-Wno-unused-result",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
+    "-mllvm", "-simplifycfg-sink-common=false",
     f"-I{ROOT / 'include'}",
 ]
 CXX_FLAGS = ["-std=c++17", "-O2", "-fPIC", "-Wall", f"-I{ROOT / 'include'}"]

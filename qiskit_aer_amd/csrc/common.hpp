@@ -118,7 +118,8 @@ struct PassDesc {
   uint32_t op_begin;  // first TileOp of the pass
   uint32_t nops;
   uint32_t ncomp;     // number of qubits outside the tile
-  uint32_t pad[2];
+  uint32_t readonly;  // 1: the pass only reads the state (probes); nothing is written back
+  uint32_t pad;
   uint8_t spos[16];   // tile qubits, ascending
   uint8_t comp[64];   // qubits outside the tile, ascending
 };

@@ -27,6 +27,10 @@
 
 // Sets the thread-local message returned by aerhip_last_error (engine.hip).
 extern "C" __attribute__((visibility("hidden"))) void aerhip_detail_set_error(const char *msg);
+extern "C" __attribute__((visibility("hidden"))) aerhip_qv *aerhip_detail_qv_create_planner(
+    uint64_t num_qubits, int precision, int device);
+extern "C" __attribute__((visibility("hidden"))) int aerhip_detail_batch_run(
+    aerhip_qv *const *qvs, uint64_t count, int device, double *values, double *kernel_ms);
 
 namespace {
 
@@ -529,6 +533,92 @@ int aerhip_apply_ops(aerhip_qv *qv, const aerhip_op *ops, uint64_t num_ops) {
     aerhip_detail_set_error(e.what());
     return 2;
   } catch (const std::exception &e) {
+    aerhip_detail_set_error(e.what());
+    return 1;
+  }
+}
+
+int aerhip_batch_pauli_expval(const aerhip_op *ops, const uint64_t *op_offsets,
+                              const uint32_t *num_qubits, uint64_t num_circuits,
+                              const char *labels, const double *coeffs,
+                              const uint64_t *term_offsets, int precision, int device,
+                              int mode, double *energies, double *kernel_ms) {
+  std::vector<aerhip_qv *> handles;
+  auto release = [&] {
+    for (aerhip_qv *h : handles) aerhip_qv_destroy(h);
+    handles.clear();
+  };
+  try {
+    if (!num_circuits) return 0;
+    if (!op_offsets || !num_qubits || !term_offsets || !energies)
+      throw std::invalid_argument("null argument");
+    if (op_offsets[num_circuits] && !ops) throw std::invalid_argument("null op list");
+    if (term_offsets[num_circuits] && (!labels || !coeffs)) throw std::invalid_argument("null Pauli sum");
+    if (mode != 0 && mode != 1) throw std::invalid_argument("mode must be 0 or 1");
+    // Character offset of each circuit's first label.
+    std::vector<uint64_t> label_at(num_circuits + 1, 0);
+    for (uint64_t i = 0; i < num_circuits; ++i) {
+      if (op_offsets[i + 1] < op_offsets[i] || term_offsets[i + 1] < term_offsets[i])
+        throw std::invalid_argument("offsets must not decrease");
+      label_at[i + 1] = label_at[i] + (term_offsets[i + 1] - term_offsets[i]) * num_qubits[i];
+    }
+    auto queue_circuit = [&](aerhip_qv *qv, uint64_t i) {
+      ck(aerhip_apply_ops(qv, ops + op_offsets[i], op_offsets[i + 1] - op_offsets[i]));
+      const uint64_t n = num_qubits[i];
+      std::vector<uint64_t> all(n);
+      for (uint64_t q = 0; q < n; ++q) all[q] = q;
+      for (uint64_t t = term_offsets[i]; t < term_offsets[i + 1]; ++t) {
+        const std::string lab(labels + label_at[i] + (t - term_offsets[i]) * n, n);
+        uint64_t slot = 0;
+        ck(aerhip_save_expval_pauli(qv, all.data(), n, lab.c_str(), 1.0, 0.0, &slot));
+      }
+    };
+    auto energy = [&](uint64_t i, const double *vals) {
+      double e = 0;
+      for (uint64_t t = term_offsets[i]; t < term_offsets[i + 1]; ++t)
+        e += coeffs[t] * vals[t - term_offsets[i]];
+      return e;
+    };
+    if (mode == 0) {
+      handles.reserve(num_circuits);
+      for (uint64_t i = 0; i < num_circuits; ++i) {
+        aerhip_qv *h = aerhip_detail_qv_create_planner(num_qubits[i], precision, device);
+        if (!h) throw std::runtime_error(aerhip_last_error());
+        handles.push_back(h);
+        queue_circuit(h, i);
+      }
+      std::vector<double> vals(term_offsets[num_circuits] - term_offsets[0] + 1);
+      ck(aerhip_detail_batch_run(handles.data(), num_circuits, device, vals.data(), kernel_ms));
+      for (uint64_t i = 0; i < num_circuits; ++i)
+        energies[i] = energy(i, vals.data() + (term_offsets[i] - term_offsets[0]));
+    } else {
+      // One statevector per distinct size, reused circuit after circuit.
+      std::map<uint64_t, aerhip_qv *> by_n;
+      std::vector<double> vals;
+      for (uint64_t i = 0; i < num_circuits; ++i) {
+        aerhip_qv *&h = by_n[num_qubits[i]];
+        if (!h) {
+          h = aerhip_qv_create(num_qubits[i], precision, device);
+          if (!h) throw std::runtime_error(aerhip_last_error());
+          handles.push_back(h);
+        }
+        ck(aerhip_qv_initialize(h));
+        queue_circuit(h, i);
+        vals.assign(term_offsets[i + 1] - term_offsets[i] + 1, 0.0);
+        uint64_t count = 0;
+        ck(aerhip_saved_expvals(h, vals.data(), vals.size(), &count));
+        energies[i] = energy(i, vals.data());
+      }
+      if (kernel_ms) *kernel_ms = 0;
+    }
+    release();
+    return 0;
+  } catch (const std::invalid_argument &e) {
+    release();
+    aerhip_detail_set_error(e.what());
+    return 2;
+  } catch (const std::exception &e) {
+    release();
     aerhip_detail_set_error(e.what());
     return 1;
   }

@@ -9,6 +9,7 @@
 #include "engine.hpp"
 #include "gate_matrices.hpp"
 #include "kernels.hpp"
+#include "batch.hpp"
 #include "reg_pass.hpp"
 #include "tile_pass.hpp"
 
@@ -53,7 +54,20 @@ Engine::Engine(uint32_t num_qubits, int precision, int device)
   AERHIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&h_scalar_), 64 * sizeof(double)));
 }
 
+Engine::Engine(uint32_t num_qubits, int precision, int device, PlanOnly)
+    : n_(num_qubits), prec_(precision), device_(device), plan_only_(true) {
+  if (precision != AERHIP_PRECISION_DOUBLE && precision != AERHIP_PRECISION_SINGLE)
+    throw std::invalid_argument("aerhip: invalid precision");
+  if (num_qubits < 1 || num_qubits > 40)
+    throw std::invalid_argument("aerhip: number of qubits must be in [1, 40]");
+  size_ = 1ull << n_;
+  tile_bits_ = std::min<uint32_t>(n_, prec_ == AERHIP_PRECISION_DOUBLE ? kTileBitsC128 : kTileBitsC64);
+  low_bits_ = std::min<uint32_t>(n_, prec_ == AERHIP_PRECISION_DOUBLE ? 3 : 4);
+  reg_bits_ = prec_ == AERHIP_PRECISION_DOUBLE ? 3 : 4;  // 256 threads: the batch kernel's shape
+}
+
 Engine::~Engine() {
+  if (plan_only_) return;
   (void)hipSetDevice(device_);
   if (stream_) (void)hipStreamSynchronize(stream_);
   if (sv_) (void)hipFree(sv_);
@@ -235,7 +249,7 @@ void Engine::push(Prim &&p) {
   if (!p.unitary) p.touch = ~0ull;
   if (p.kind != K_PROBE) stats_.gates++;
   queue_.push_back(std::move(p));
-  if (queue_.size() >= kMaxQueue) flush();
+  if (queue_.size() >= kMaxQueue && !plan_only_) flush();
 }
 
 namespace {
@@ -475,6 +489,21 @@ void Engine::apply_pauli(const std::vector<uint32_t> &qubits, const std::string 
   if (prec_ == AERHIP_PRECISION_SINGLE)  // the phase is formed in the state precision
     ph = cd(static_cast<float>(coeff.real()), static_cast<float>(coeff.imag()));
   ph = add_y_phase(m.num_y, ph);
+  const uint64_t lowmask = (1ull << low_bits_) - 1ull;
+  if (static_cast<uint32_t>(__builtin_popcountll(m.x | lowmask)) > tile_bits_) {
+    // X/Y on more qubits than a tile holds: the sign-and-phase part as a
+    // Z-only string, then one X per qubit (the values are the same: each
+    // amplitude is multiplied by the same +-phase and moved).
+    Prim d;
+    d.kind = K_PAULI;
+    d.zmask = m.z;
+    d.pr = ph.real();
+    d.pi = ph.imag();
+    push(std::move(d));
+    for (uint32_t q = 0; q < n_; ++q)
+      if (m.x & bit(q)) apply_mcx({q});
+    return;
+  }
   Prim p;
   p.kind = K_PAULI;
   p.xmask = m.x;
@@ -725,6 +754,10 @@ void validate_register_program(const ROp<double> *p, size_t n, uint32_t B, uint3
         case Q_U2:
           if (r.a >= R || r.b >= R || r.a == r.b) fail("register bits of a 2-qubit gate");
           break;
+        case Q_XR:
+          if (r.a >= R || (r.b != 0xFFu && (r.b >= R || r.b == r.a)) || r.extra > 1)
+            fail("register-controlled X");
+          break;
         case Q_D1: if (r.a >= B) fail("diagonal bit"); break;
         case Q_DIAG: if (r.extra >= ntops) fail("diagonal index"); break;
         case Q_PROBE: if (r.u.pr.xr >= (1u << R)) fail("probe mask"); break;
@@ -858,6 +891,13 @@ void register_program(const std::vector<TileOp> &in, uint32_t B, uint32_t R, uin
       case K_MCY:
         r.kind = op.kind == K_MCX ? Q_X : Q_Y;
         r.a = regidx(rs, op.t);
+        // X with at most one control, on a register bit: register moves only
+        if (op.kind == K_MCX && op.cg == 0 && (op.cl & ~rs) == 0 && __builtin_popcount(op.cl) <= 1) {
+          r.kind = Q_XR;
+          r.b = op.cl ? regidx(rs, static_cast<uint32_t>(__builtin_ctz(op.cl))) : 0xFFu;
+          r.extra = (op.cvl & op.cl) ? 1u : 0u;
+          r.cl = r.cvl = 0;
+        }
         break;
       case K_DENSE:
         r.kind = Q_U2;
@@ -974,6 +1014,9 @@ std::vector<PassPlan> Engine::plan() const {
   const uint32_t nb = tile_bits_;
   const uint64_t lowmask = (1ull << low_bits_) - 1ull;
   size_t seg_begin = 0;
+  for (const Prim &p : queue_)
+    if (!p.solo && static_cast<uint32_t>(__builtin_popcountll(p.tmask | lowmask)) > nb)
+      throw std::logic_error("aerhip: a primitive mixes more qubits than a tile holds");
   while (seg_begin < queue_.size()) {
     size_t seg_end = seg_begin;
     while (seg_end < queue_.size() && !queue_[seg_end].solo) ++seg_end;
@@ -1021,15 +1064,18 @@ std::vector<PassPlan> Engine::plan() const {
   return out;
 }
 
-void Engine::flush() {
-  if (queue_.empty()) return;
-  AERHIP_CHECK(hipSetDevice(device_));
+// Turns the queue into passes: PassDescs, TileOps (LDS passes and the LDS
+// primitives of register passes), ROps (register passes) and the coefficient
+// pool, appended to `prog` (pool offsets are relative to prog.pool).  Consumes
+// the queue.
+void Engine::emit_program(Program &prog) {
   const std::vector<PassPlan> plans = plan();
-
-  std::vector<PassDesc> passes;
-  std::vector<TileOp> ops;
-  std::vector<ROp<double>> rops;
-  passes.reserve(plans.size());
+  std::vector<PassDesc> &passes = prog.passes;
+  std::vector<TileOp> &ops = prog.ops;
+  std::vector<ROp<double>> &rops = prog.rops;
+  const uint32_t pool_base = static_cast<uint32_t>(prog.pool.size());
+  prog.pool.insert(prog.pool.end(), pool_.begin(), pool_.end());
+  passes.reserve(passes.size() + plans.size());
   for (const PassPlan &pp : plans) {
     PassDesc pd;
     std::memset(&pd, 0, sizeof(pd));
@@ -1045,7 +1091,34 @@ void Engine::flush() {
       }
     }
     pd.kind = pp.solo ? 1u : 0u;
+    if (pp.solo && queue_[pp.prims[0]].kind == K_PROBE) {
+      // probe sweep: one TileOp with the masks over all qubits
+      const Prim &p = queue_[pp.prims[0]];
+      TileOp op;
+      std::memset(&op, 0, sizeof(op));
+      op.kind = K_PROBE;
+      op.cg = p.xmask;
+      op.zg = p.zmask;
+      op.t = p.q.empty() ? 0u : p.q[0];
+      op.npairs = p.npairs;
+      op.pr = p.pr;
+      op.pi = p.pi;
+      PassDesc sd;
+      std::memset(&sd, 0, sizeof(sd));
+      sd.kind = 3;
+      sd.readonly = 1;
+      sd.nb = tile_bits_;
+      sd.ncomp = n_ - tile_bits_;
+      sd.op_begin = static_cast<uint32_t>(ops.size());
+      sd.nops = 1;
+      ops.push_back(op);
+      passes.push_back(sd);
+      continue;
+    }
     pd.nb = ns;
+    pd.readonly = 1;
+    for (uint32_t idx : pp.prims)
+      if (queue_[idx].kind != K_PROBE) pd.readonly = 0;
     uint32_t c = 0;
     while (c < ns && pd.spos[c] == c) ++c;
     pd.c = c;
@@ -1057,7 +1130,7 @@ void Engine::flush() {
       TileOp op;
       std::memset(&op, 0, sizeof(op));
       op.kind = p.kind;
-      op.off = p.off;
+      op.off = p.off + pool_base;
       op.npairs = p.npairs;
       op.pr = p.pr;
       op.pi = p.pi;
@@ -1107,8 +1180,9 @@ void Engine::flush() {
     if (!pp.solo && reg_bits_ > 0 && ns == full_bits) {
       pd.kind = 2;  // register-layout tile pass: op_begin/nops index the ROps
       pd.op_begin = static_cast<uint32_t>(rops.size());
-      register_program(pops, ns, reg_bits_, low_bits_, pd.spos, pool_, rops, ops, &stats_.gathers);
+      register_program(pops, ns, reg_bits_, low_bits_, pd.spos, prog.pool, rops, ops, &stats_.gathers);
       pd.nops = static_cast<uint32_t>(rops.size()) - pd.op_begin;
+      if (pd.readonly) rops.back().extra = 1;  // the final STORE writes nothing
       validate_register_program(rops.data() + pd.op_begin, pd.nops, ns, reg_bits_, pd.spos, ops.size());
     } else {
       pd.nops = static_cast<uint32_t>(pops.size());
@@ -1117,6 +1191,22 @@ void Engine::flush() {
     passes.push_back(pd);
   }
 
+  queue_.clear();
+  pool_.clear();
+}
+
+void Engine::flush() {
+  if (queue_.empty()) return;
+  if (plan_only_) throw std::logic_error("aerhip: a planning handle cannot execute");
+  AERHIP_CHECK(hipSetDevice(device_));
+  const uint64_t nprobes = probes_queued_;
+  Program prog;
+  emit_program(prog);
+  const std::vector<PassDesc> &passes = prog.passes;
+  const std::vector<TileOp> &ops = prog.ops;
+  const std::vector<ROp<double>> &rops = prog.rops;
+  const std::vector<cd> &pool = prog.pool;
+
   // Upload [passes | ops | rops | pool] with one copy from pinned staging.
   const size_t pass_bytes = passes.size() * sizeof(PassDesc);
   const size_t ops_off = align_up(pass_bytes, 256);
@@ -1124,7 +1214,7 @@ void Engine::flush() {
   const size_t rops_off = align_up(ops_off + ops_bytes, 256);
   const size_t rops_bytes = rops.size() * sizeof(ROp<double>);
   const size_t pool_off = align_up(rops_off + rops_bytes, 256);
-  const size_t total = pool_off + pool_.size() * sizeof(cd) + 16;
+  const size_t total = pool_off + pool.size() * sizeof(cd) + 16;
   AERHIP_CHECK(hipEventSynchronize(prog_ev_));  // previous upload finished with staging
   if (total > h_prog_cap_) {
     if (h_prog_) AERHIP_CHECK(hipHostFree(h_prog_));
@@ -1144,15 +1234,9 @@ void Engine::flush() {
     std::memcpy(h + rops_off, rops.data(), rops_bytes);
   } else {
     // complex64 programs carry their matrices in float
-    auto *rf = reinterpret_cast<ROp<float> *>(h + rops_off);
-    for (size_t i = 0; i < rops.size(); ++i) {
-      std::memcpy(&rf[i], &rops[i], sizeof(ROp<float>));
-      const uint32_t k = rops[i].kind;
-      if (k == Q_U1 || k == Q_U1C || k == Q_D1 || k == Q_PHASE)
-        for (int e = 0; e < 8; ++e) rf[i].u.m[e] = static_cast<float>(rops[i].u.m[e]);
-    }
+    rops_to_float(rops.data(), rops.size(), reinterpret_cast<ROp<float> *>(h + rops_off));
   }
-  if (!pool_.empty()) std::memcpy(h + pool_off, pool_.data(), pool_.size() * sizeof(cd));
+  if (!pool.empty()) std::memcpy(h + pool_off, pool.data(), pool.size() * sizeof(cd));
   AERHIP_CHECK(hipMemcpyAsync(d_prog_, h_prog_, total, hipMemcpyHostToDevice, stream_));
   AERHIP_CHECK(hipEventRecord(prog_ev_, stream_));
 
@@ -1164,7 +1248,6 @@ void Engine::flush() {
   // Probe partials: one double per (probe, tile); every tile pass has
   // 2^(n - tile_bits) tiles.
   double *dprobes = nullptr;
-  const uint64_t nprobes = probes_queued_;
   const uint64_t ntiles = 1ull << (n_ - tile_bits_);
   if (nprobes) {
     const size_t need = (nprobes * ntiles + nprobes) * sizeof(double);
@@ -1200,14 +1283,24 @@ void Engine::flush() {
           k_reg_pass<float, kTileBitsC64, 5><<<grid, 1 << (kTileBitsC64 - 5), 0, stream_>>>(
               sv, dpass + i, ro, dops, dpool, dprobes);
       }
+    } else if (pd.kind == 3) {
+      // probe sweep over the whole state, one workgroup per tile-sized slice
+      const uint32_t grid = static_cast<uint32_t>(1ull << (n_ - tile_bits_));
+      if (prec_ == AERHIP_PRECISION_DOUBLE)
+        k_probe_sweep<double><<<grid, kThreads, 0, stream_>>>(static_cast<const cx<double> *>(sv_),
+                                                              size_, dops + pd.op_begin, dprobes);
+      else
+        k_probe_sweep<float><<<grid, kThreads, 0, stream_>>>(static_cast<const cx<float> *>(sv_),
+                                                             size_, dops + pd.op_begin, dprobes);
     } else if (pd.kind == 0) {
-      // LDS-resident tile pass: the same kernel without register layouts.
+      // LDS-resident tile pass (tiles smaller than the register layouts need,
+      // or register layouts switched off).
       const uint32_t grid = static_cast<uint32_t>(1ull << pd.ncomp);
       if (prec_ == AERHIP_PRECISION_DOUBLE)
-        k_tile_reg<double, kTileBitsC128, 3><<<grid, kThreads, 0, stream_>>>(
+        k_tile_lds<double, kTileBitsC128><<<grid, kThreads, 0, stream_>>>(
             static_cast<cx<double> *>(sv_), dpass + i, dops, dpool, dprobes);
       else
-        k_tile_reg<float, kTileBitsC64, 4><<<grid, kThreads, 0, stream_>>>(
+        k_tile_lds<float, kTileBitsC64><<<grid, kThreads, 0, stream_>>>(
             static_cast<cx<float> *>(sv_), dpass + i, dops, dpool, dprobes);
     } else {
       const uint32_t grid = static_cast<uint32_t>(1ull << (n_ - pd.nb));
@@ -1243,8 +1336,6 @@ void Engine::flush() {
     stats_.pass_ms += ms;
     stats_.timed_launches += passes.size();
   }
-  queue_.clear();
-  pool_.clear();
 }
 
 void Engine::set_reg_bits(uint32_t r) {
@@ -1305,6 +1396,35 @@ double Engine::norm() {
   flush();
   AERHIP_CHECK(hipSetDevice(device_));
   return prec_ == AERHIP_PRECISION_DOUBLE ? norm_t<double>() : norm_t<float>();
+}
+
+cd Engine::inner_product(Engine &other) {
+  if (other.n_ != n_) throw std::invalid_argument("aerhip: inner product of states of different size");
+  if (other.device_ != device_) throw std::invalid_argument("aerhip: inner product across devices");
+  flush();
+  other.flush();
+  AERHIP_CHECK(hipStreamSynchronize(other.stream_));
+  AERHIP_CHECK(hipSetDevice(device_));
+  const uint32_t grid = grid_for(size_, 4096);
+  ensure_partial(2 * grid + 2);
+  const bool a64 = prec_ == AERHIP_PRECISION_DOUBLE, b64 = other.prec_ == AERHIP_PRECISION_DOUBLE;
+  if (a64 && b64)
+    k_inner<double, double><<<grid, kThreads, 0, stream_>>>(static_cast<const cx<double> *>(sv_),
+        static_cast<const cx<double> *>(other.sv_), size_, d_partial_);
+  else if (a64)
+    k_inner<double, float><<<grid, kThreads, 0, stream_>>>(static_cast<const cx<double> *>(sv_),
+        static_cast<const cx<float> *>(other.sv_), size_, d_partial_);
+  else if (b64)
+    k_inner<float, double><<<grid, kThreads, 0, stream_>>>(static_cast<const cx<float> *>(sv_),
+        static_cast<const cx<double> *>(other.sv_), size_, d_partial_);
+  else
+    k_inner<float, float><<<grid, kThreads, 0, stream_>>>(static_cast<const cx<float> *>(sv_),
+        static_cast<const cx<float> *>(other.sv_), size_, d_partial_);
+  k_sum_rows<<<1, kThreads, 0, stream_>>>(d_partial_, 2, grid, d_partial_ + 2 * grid);
+  AERHIP_CHECK(hipGetLastError());
+  const double re = fetch_scalar(d_partial_ + 2 * grid);
+  const double im = fetch_scalar(d_partial_ + 2 * grid + 1);
+  return cd(re, im);
 }
 
 double Engine::probability(uint64_t outcome) {
@@ -1475,6 +1595,9 @@ uint64_t Engine::save_expval_pauli(const std::vector<uint32_t> &qubits, const st
   p.zmask = m.z;
   p.tmask = m.x;
   if (m.x) p.q = {m.x_max};
+  // X/Y on more qubits than a tile holds: a sweep of its own over the state.
+  const uint64_t lowmask = (1ull << low_bits_) - 1ull;
+  p.solo = static_cast<uint32_t>(__builtin_popcountll(m.x | lowmask)) > tile_bits_;
   p.npairs = static_cast<uint32_t>(probes_queued_++);
   push(std::move(p));
   return slot;
@@ -1532,6 +1655,119 @@ double Engine::expval_pauli(const std::vector<uint32_t> &qubits, const std::stri
 // ===========================================================================
 // C ABI
 // ===========================================================================
+
+namespace aerhip {
+
+// Runs the queues of `count` planning engines (same precision) as one batch:
+// every circuit starts from |0>; the saved expectation values of circuit i
+// are written to values[offset_i ..], in circuit order and slot order.
+// `kernel_ms` receives the device time of the batch kernel.
+void run_batch(Engine *const *engines, uint64_t count, int device, double *values,
+               double *kernel_ms) {
+  if (!count) return;
+  if (!engines || !values) throw std::invalid_argument("aerhip: null argument");
+  const int prec = engines[0]->precision();
+  Program prog;
+  std::vector<BatchCircuit> circ(count);
+  std::vector<double> cost(count);
+  uint64_t sv_off = 0, probe_off = 0;
+  for (uint64_t i = 0; i < count; ++i) {
+    Engine &e = *engines[i];
+    if (!e.plan_only() || e.precision() != prec)
+      throw std::invalid_argument("aerhip: batch needs planning handles of one precision");
+    BatchCircuit &c = circ[i];
+    std::memset(&c, 0, sizeof(c));
+    c.sv_off = sv_off;
+    c.num_qubits = e.num_qubits();
+    c.pass_begin = static_cast<uint32_t>(prog.passes.size());
+    c.probe_base = static_cast<uint32_t>(probe_off);
+    c.num_probes = static_cast<uint32_t>(e.probes_queued());
+    e.emit_program(prog);
+    c.pass_count = static_cast<uint32_t>(prog.passes.size()) - c.pass_begin;
+    for (uint32_t p = c.pass_begin; p < prog.passes.size(); ++p)
+      if (prog.passes[p].kind == 1)
+        throw std::invalid_argument("aerhip: batched circuits take dense matrices on at most 5 qubits");
+    cost[i] = static_cast<double>(c.pass_count + 1) * static_cast<double>(1ull << c.num_qubits);
+    sv_off += align_up(1ull << c.num_qubits, 256);
+    probe_off += c.num_probes;
+  }
+  // Largest circuits first: workgroups start in index order.
+  std::vector<uint64_t> order(count);
+  for (uint64_t i = 0; i < count; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](uint64_t a, uint64_t b) { return cost[a] > cost[b]; });
+  std::vector<BatchCircuit> sorted(count);
+  for (uint64_t i = 0; i < count; ++i) sorted[i] = circ[order[i]];
+
+  const size_t amp = prec == AERHIP_PRECISION_DOUBLE ? 16 : 8;
+  const size_t circ_bytes = count * sizeof(BatchCircuit);
+  const size_t pass_off = align_up(circ_bytes, 256);
+  const size_t ops_off = align_up(pass_off + prog.passes.size() * sizeof(PassDesc), 256);
+  const size_t rops_off = align_up(ops_off + prog.ops.size() * sizeof(TileOp), 256);
+  const size_t pool_off = align_up(rops_off + prog.rops.size() * sizeof(ROp<double>), 256);
+  const size_t total = pool_off + prog.pool.size() * sizeof(cd) + 16;
+  std::vector<char> h(total, 0);
+  std::memcpy(h.data(), sorted.data(), circ_bytes);
+  std::memcpy(h.data() + pass_off, prog.passes.data(), prog.passes.size() * sizeof(PassDesc));
+  std::memcpy(h.data() + ops_off, prog.ops.data(), prog.ops.size() * sizeof(TileOp));
+  if (prec == AERHIP_PRECISION_DOUBLE) {
+    std::memcpy(h.data() + rops_off, prog.rops.data(), prog.rops.size() * sizeof(ROp<double>));
+  } else {
+    rops_to_float(prog.rops.data(), prog.rops.size(), reinterpret_cast<ROp<float> *>(h.data() + rops_off));
+  }
+  if (!prog.pool.empty()) std::memcpy(h.data() + pool_off, prog.pool.data(), prog.pool.size() * sizeof(cd));
+
+  AERHIP_CHECK(hipSetDevice(device));
+  void *d_states = nullptr, *d_prog = nullptr;
+  double *d_probes = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  auto release = [&] {
+    if (st) (void)hipStreamSynchronize(st);
+    if (d_states) (void)hipFree(d_states);
+    if (d_prog) (void)hipFree(d_prog);
+    if (d_probes) (void)hipFree(d_probes);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+  };
+  try {
+    AERHIP_CHECK(hipStreamCreate(&st));
+    AERHIP_CHECK(hipEventCreate(&e0));
+    AERHIP_CHECK(hipEventCreate(&e1));
+    AERHIP_CHECK(hipMalloc(&d_states, sv_off * amp));
+    AERHIP_CHECK(hipMalloc(&d_prog, total));
+    AERHIP_CHECK(hipMalloc(reinterpret_cast<void **>(&d_probes), (probe_off + 1) * sizeof(double)));
+    AERHIP_CHECK(hipMemcpyAsync(d_prog, h.data(), total, hipMemcpyHostToDevice, st));
+    char *d = static_cast<char *>(d_prog);
+    const auto *dc = reinterpret_cast<const BatchCircuit *>(d);
+    const auto *dp = reinterpret_cast<const PassDesc *>(d + pass_off);
+    const auto *dops = reinterpret_cast<const TileOp *>(d + ops_off);
+    const auto *dpool = reinterpret_cast<const cx<double> *>(d + pool_off);
+    const uint32_t grid = static_cast<uint32_t>(count);
+    AERHIP_CHECK(hipEventRecord(e0, st));
+    if (prec == AERHIP_PRECISION_DOUBLE)
+      k_batch<double, kTileBitsC128, 3><<<grid, kThreads, 0, st>>>(
+          static_cast<cx<double> *>(d_states), dc, dp,
+          reinterpret_cast<const ROp<double> *>(d + rops_off), dops, dpool, d_probes);
+    else
+      k_batch<float, kTileBitsC64, 4><<<grid, kThreads, 0, st>>>(
+          static_cast<cx<float> *>(d_states), dc, dp,
+          reinterpret_cast<const ROp<float> *>(d + rops_off), dops, dpool, d_probes);
+    AERHIP_CHECK(hipGetLastError());
+    AERHIP_CHECK(hipEventRecord(e1, st));
+    AERHIP_CHECK(hipMemcpyAsync(values, d_probes, probe_off * sizeof(double), hipMemcpyDeviceToHost, st));
+    AERHIP_CHECK(hipStreamSynchronize(st));
+    float ms = 0;
+    AERHIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    if (kernel_ms) *kernel_ms = ms;
+  } catch (...) {
+    release();
+    throw;
+  }
+  release();
+}
+
+}  // namespace aerhip
 
 using aerhip::cd;
 using aerhip::Engine;
@@ -1600,6 +1836,34 @@ int aerhip_device_count(void) {
 }
 
 const char *aerhip_version(void) { return "aerhip 0.1.0 (gfx950)"; }
+
+// Internal to the library: planning handles and the batch runner used by
+// aerhip_batch_pauli_expval (controller.cpp).
+__attribute__((visibility("hidden"))) aerhip_qv *aerhip_detail_qv_create_planner(
+    uint64_t num_qubits, int precision, int device) {
+  aerhip_qv *h = nullptr;
+  const int rc = guard([&] {
+    auto eng = std::make_unique<Engine>(static_cast<uint32_t>(num_qubits), precision, device,
+                                        Engine::PlanOnly{});
+    h = new aerhip_qv{std::move(eng)};
+  });
+  return rc == 0 ? h : nullptr;
+}
+
+// Runs the queues of `count` planning handles (same precision) as one batch
+// (aerhip::run_batch).
+__attribute__((visibility("hidden"))) int aerhip_detail_batch_run(aerhip_qv *const *qvs,
+                                                                   uint64_t count, int device,
+                                                                   double *values,
+                                                                   double *kernel_ms) {
+  return guard([&] {
+    if (!count) return;
+    if (!qvs) throw std::invalid_argument("aerhip: null argument");
+    std::vector<Engine *> engines(count);
+    for (uint64_t i = 0; i < count; ++i) engines[i] = &E(qvs[i]);
+    aerhip::run_batch(engines.data(), count, device, values, kernel_ms);
+  });
+}
 
 aerhip_qv *aerhip_qv_create(uint64_t num_qubits, int precision, int device) {
   aerhip_qv *h = nullptr;
@@ -1797,6 +2061,15 @@ int aerhip_apply_multiplexer(aerhip_qv *qv, const uint64_t *controls, uint64_t n
 }
 int aerhip_apply_multi_swaps(aerhip_qv *qv, const uint64_t *qubits, uint64_t nq) {
   return guard([&] { E(qv).apply_multi_swaps(qvec(qubits, nq)); });
+}
+
+int aerhip_qv_inner_product(aerhip_qv *a, aerhip_qv *b, double *re, double *im) {
+  return guard([&] {
+    if (!a || !b || !re || !im) throw std::invalid_argument("aerhip: null argument");
+    const cd v = E(a).inner_product(E(b));
+    *re = v.real();
+    *im = v.imag();
+  });
 }
 
 int aerhip_norm(aerhip_qv *qv, double *out) {

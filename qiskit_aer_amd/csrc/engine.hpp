@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "reg_pass.hpp"
 
 namespace aerhip {
 
@@ -36,6 +37,14 @@ struct PassPlan {
   std::vector<uint32_t> prims; // indices into the queue
 };
 
+// A flush's device program before upload: pool offsets index `pool`.
+struct Program {
+  std::vector<PassDesc> passes;
+  std::vector<TileOp> ops;
+  std::vector<ROp<double>> rops;
+  std::vector<cd> pool;
+};
+
 struct EngineStats {
   uint64_t passes = 0;
   uint64_t launches = 0;
@@ -48,6 +57,10 @@ struct EngineStats {
 class Engine {
  public:
   Engine(uint32_t num_qubits, int precision, int device);
+  // A handle that only queues and plans (no device state, never executes):
+  // the batch runner turns its queue into a program with emit_program().
+  struct PlanOnly {};
+  Engine(uint32_t num_qubits, int precision, int device, PlanOnly);
   ~Engine();
   Engine(const Engine &) = delete;
   Engine &operator=(const Engine &) = delete;
@@ -96,6 +109,8 @@ class Engine {
   std::vector<double> probabilities(const std::vector<uint32_t> &qubits);
   std::vector<uint64_t> sample_measure(const std::vector<double> &rnds);
   double expval_pauli(const std::vector<uint32_t> &qubits, const std::string &pauli, cd phase);
+  // <this|other>, accumulated in FP64; the states may differ in precision.
+  cd inner_product(Engine &other);
 
   // Queues Re(<psi| phase * P |psi>) at the current point of the gate
   // sequence (evaluated inside the tile passes); returns its slot.
@@ -110,6 +125,10 @@ class Engine {
 
   // Executes the queue.
   void flush();
+  // Appends the queue's passes to `prog` and clears the queue.
+  void emit_program(Program &prog);
+  bool plan_only() const { return plan_only_; }
+  uint64_t probes_queued() const { return probes_queued_; }
 
  private:
   void check_qubits(const std::vector<uint32_t> &qubits) const;
@@ -134,6 +153,7 @@ class Engine {
   uint32_t n_;
   int prec_;
   int device_;
+  bool plan_only_ = false;
   uint64_t size_;
   void *sv_ = nullptr;
   hipStream_t stream_ = nullptr;

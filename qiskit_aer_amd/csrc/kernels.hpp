@@ -156,6 +156,27 @@ __global__ __launch_bounds__(kThreads) void k_sumsq(const cx<T> *__restrict__ sv
   if (threadIdx.x == 0) partial[blockIdx.x] = acc;
 }
 
+// <a|b> = sum_i conj(a_i) b_i in FP64: real parts to partial[block], imaginary
+// parts to partial[gridDim.x + block].
+template <typename TA, typename TB>
+__global__ __launch_bounds__(kThreads) void k_inner(const cx<TA> *__restrict__ a,
+                                                  const cx<TB> *__restrict__ b, uint64_t n,
+                                                  double *__restrict__ partial) {
+  double re = 0, im = 0;
+  for (uint64_t i = blockIdx.x * (uint64_t)kThreads + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * kThreads) {
+    const double ax = a[i].x, ay = a[i].y, bx = b[i].x, by = b[i].y;
+    re += ax * bx + ay * by;
+    im += ax * by - ay * bx;
+  }
+  re = block_sum(re);
+  im = block_sum(im);
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = re;
+    partial[gridDim.x + blockIdx.x] = im;
+  }
+}
+
 // Sums `cols` consecutive partials of each of `rows` rows (one thread per row,
 // sequential order).
 __global__ void k_sum_rows(const double *__restrict__ partial, uint64_t rows, uint32_t cols,

@@ -19,13 +19,16 @@
 // applied to the tile in LDS (Q_SPILL, TileOp primitives, Q_FILL).
 #pragma once
 
+#include <cstring>
+#include <utility>
+
 #include "tile_pass.hpp"
 
 namespace aerhip {
 
 enum RegPassOpKind : uint32_t {
   Q_LOAD = 40,      // HBM -> registers in layout u.lay; extra: register ops that follow
-  Q_STORE = 41,     // registers -> HBM in layout u.lay (the current layout)
+  Q_STORE = 41,     // registers -> HBM in layout u.lay (extra = 1: read-only pass, no write)
   Q_RELAYOUT = 42,  // registers -> LDS -> registers in layout u.lay
   Q_SPILL = 43,     // registers -> LDS (the tile is then in LDS)
   Q_FILL = 44,      // LDS -> registers in layout u.lay; extra: register ops that follow
@@ -34,12 +37,14 @@ enum RegPassOpKind : uint32_t {
   Q_U1C = 47,       // 2x2 matrix on register bit a where (l & cl) == cvl
   Q_X = 48,         // X on register bit a where (l & cl) == cvl
   Q_Y = 49,         // Y on register bit a where (l & cl) == cvl
-  Q_U2 = 50,        // 4x4 matrix pool[extra..] on register bits a (bit 0), b (bit 1)
+  Q_U2 = 50,        // 4x4 matrix pool[extra..] on register bits a (matrix bit 0), b (bit 1)
   Q_D1 = 51,        // diag(u.m[0..1], u.m[2..3]) on local bit a where (l & cl) == cvl
   Q_PHASE = 52,     // multiply by u.m[0..1] where (l & cl) == cvl
   Q_DIAG = 53,      // diagonal of TileOp extra (local and fixed bits)
   Q_PROBE = 54,     // Pauli probe whose X/Y bits are register bits (u.pr)
   Q_NOP = 55,       // keeps the amplitudes (pads a register phase to an even length)
+  Q_XR = 56,        // X on register bit a controlled by register bit b with value extra
+                    // (b = 0xFF: no control): register moves only
 };
 
 // Bit positions packed four to a word (byte i of word w is entry 4w+i), so
@@ -65,6 +70,8 @@ struct RProbe {
   double pr, pi;
 };
 
+// 128-byte op record: a 64-byte header and a 64-byte payload, each read with
+// one scalar load (both issued together, so an op costs one load latency).
 template <typename T>
 struct alignas(128) ROp {
   uint32_t kind;
@@ -72,25 +79,53 @@ struct alignas(128) ROp {
   uint32_t cl, cvl;  // local control mask and value
   uint32_t extra;    // probe slot, TileOp index or pool offset
   uint64_t cg, cvg;  // controls on qubits fixed by the tile
+  uint32_t pad[6];
   union {
-    T m[8];
+    T m[8];          // 2x2 complex matrix (column major), diagonal, phase
+    float mr[16];    // complex64 Q_U1: the matrix as a real 4x4, rows over
+                     // (re a0', im a0', re a1', im a1'), columns over (re a0, im a0, re a1, im a1)
     RLayout lay;
     RProbe pr;
   } u;
 };
 static_assert(sizeof(ROp<double>) == 128 && sizeof(ROp<float>) == 128, "ROp is one 128-byte record");
 
+// complex64 copy of a register program: matrices rounded to float; Q_U1
+// carries its matrix as the real 4x4 that the MFMA path multiplies by.
+inline void rops_to_float(const ROp<double> *in, size_t n, ROp<float> *out) {
+  for (size_t i = 0; i < n; ++i) {
+    std::memcpy(&out[i], &in[i], sizeof(ROp<float>));
+    const uint32_t k = in[i].kind;
+    if (k == Q_U1) {
+      const double *m = in[i].u.m;  // m0 = u00, m1 = u10, m2 = u01, m3 = u11 (re, im)
+      const double u00r = m[0], u00i = m[1], u10r = m[2], u10i = m[3];
+      const double u01r = m[4], u01i = m[5], u11r = m[6], u11i = m[7];
+      const double rows[16] = {u00r, -u00i, u01r, -u01i,   // re a0'
+                               u00i, u00r, u01i, u01r,     // im a0'
+                               u10r, -u10i, u11r, -u11i,   // re a1'
+                               u10i, u10r, u11i, u11r};    // im a1'
+      for (int e = 0; e < 16; ++e) out[i].u.mr[e] = static_cast<float>(rows[e]);
+    } else if (k == Q_U1C || k == Q_D1 || k == Q_PHASE) {
+      for (int e = 0; e < 8; ++e) out[i].u.m[e] = static_cast<float>(in[i].u.m[e]);
+    }
+  }
+}
+
+// Fused multiply-add in the amplitude precision.
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
 // m0*a0 + m1*a1 with one multiply and three fused multiply-adds per component.
 template <typename T>
 __device__ __forceinline__ cx<T> mv2(cx<T> m0, cx<T> a0, cx<T> m1, cx<T> a1) {
   T x = m1.x * a1.x;
-  x = __builtin_fma(-m1.y, a1.y, x);
-  x = __builtin_fma(m0.x, a0.x, x);
-  x = __builtin_fma(-m0.y, a0.y, x);
+  x = fma_t(-m1.y, a1.y, x);
+  x = fma_t(m0.x, a0.x, x);
+  x = fma_t(-m0.y, a0.y, x);
   T y = m1.x * a1.y;
-  y = __builtin_fma(m1.y, a1.x, y);
-  y = __builtin_fma(m0.x, a0.y, y);
-  y = __builtin_fma(m0.y, a0.x, y);
+  y = fma_t(m1.y, a1.x, y);
+  y = fma_t(m0.x, a0.y, y);
+  y = fma_t(m0.y, a0.x, y);
   return mk<T>(x, y);
 }
 
@@ -99,11 +134,23 @@ __device__ __forceinline__ cx<T> csel(bool c, cx<T> a, cx<T> b) {
   return mk<T>(c ? a.x : b.x, c ? a.y : b.y);
 }
 
-// Uniform layout state: rl[j] (tile index) and gr[j] (HBM offset) of register j.
+// Uniform layout state: the tile-bit mask of each register bit; the tile
+// index of register j is the OR of the masks of j's bits (at(j), folded at
+// compile time since j always is a constant).  R words instead of 2^R keep
+// the scalar register file free.
 template <int NR>
 struct RegIdx {
-  uint32_t rl[NR];
+  static constexpr int R = NR == 8 ? 3 : NR == 16 ? 4 : 5;
+  uint32_t bm[R];
+  __device__ __forceinline__ uint32_t at(int j) const {
+    uint32_t v = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if ((j >> r) & 1) v |= bm[r];
+    return v;
+  }
 };
+// HBM offset gr[j] of register j.
 template <int NR>
 struct RegOff {
   uint64_t gr[NR];
@@ -113,13 +160,7 @@ template <int R>
 __device__ __forceinline__ RegIdx<1 << R> reg_index(const RLayout &lay) {
   RegIdx<1 << R> r;
 #pragma unroll
-  for (int j = 0; j < (1 << R); ++j) {
-    uint32_t l = 0;
-#pragma unroll
-    for (int b = 0; b < R; ++b)
-      if (j & (1 << b)) l |= 1u << byte_at(lay.rp, b);
-    r.rl[j] = l;
-  }
+  for (int b = 0; b < R; ++b) r.bm[b] = 1u << byte_at(lay.rp, b);
   return r;
 }
 
@@ -153,79 +194,219 @@ __device__ __forceinline__ uint64_t thread_offset(const RLayout &lay, uint32_t t
   return g;
 }
 
-// ---- register gates, one instantiation per register bit -----------------
+// ---- register gates, one code path per register bit ----------------------
+//
+// A gate on register bit b pairs registers j and j | 2^b (b is a template
+// parameter, so every index is a compile-time constant); a small dispatch
+// picks the instantiation.  Every path writes all of its outputs, so the op
+// loop can alternate between two register banks.
 
 template <typename T, int NR, int b>
-__device__ __forceinline__ Amps<T, NR> q_u1(Amps<T, NR> a, cx<T> m0, cx<T> m1, cx<T> m2,
+__device__ __forceinline__ Amps<T, NR> q_u1(const Amps<T, NR> &a, cx<T> m0, cx<T> m1, cx<T> m2,
                                             cx<T> m3) {
+  Amps<T, NR> o;
 #pragma unroll
   for (int p = 0; p < NR / 2; ++p) {
     const int j = pair_lo<b>(p), j1 = j | (1 << b);
-    const cx<T> a0 = a.v[j], a1 = a.v[j1];
-    a.v[j] = mv2(m0, a0, m2, a1);
-    a.v[j1] = mv2(m1, a0, m3, a1);
+    o.v[j] = mv2(m0, a.v[j], m2, a.v[j1]);
+    o.v[j1] = mv2(m1, a.v[j], m3, a.v[j1]);
   }
-  return a;
+  return o;
+}
+
+// complex64 2x2 gate with MFMA: v_mfma_f32_4x4x1f32 runs 16 independent 4x4
+// blocks (lanes 4k..4k+3 form block k).  With A = row (lane % 4) of the real
+// 4x4 matrix and B = one real component of the lane's own pair, four
+// accumulating issues leave the pair's four output components in the lane's
+// accumulator: twice the FP32 rate of FMAs on the vector ALU.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+template <int NR, int b>
+__device__ __forceinline__ Amps<float, NR> q_u1_mfma(const Amps<float, NR> &a, float A0, float A1,
+                                                     float A2, float A3) {
+  Amps<float, NR> o;
+#pragma unroll
+  for (int p = 0; p < NR / 2; ++p) {
+    const int j = pair_lo<b>(p), j1 = j | (1 << b);
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    acc = __builtin_amdgcn_mfma_f32_4x4x1f32(A0, a.v[j].x, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_4x4x1f32(A1, a.v[j].y, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_4x4x1f32(A2, a.v[j1].x, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_4x4x1f32(A3, a.v[j1].y, acc, 0, 0, 0);
+    o.v[j] = mk<float>(acc[0], acc[1]);
+    o.v[j1] = mk<float>(acc[2], acc[3]);
+  }
+  return o;
 }
 
 template <typename T, int NR, int b>
-__device__ __forceinline__ Amps<T, NR> q_u1c(Amps<T, NR> a, uint32_t tlc, const RegIdx<NR> &ri,
-                                             uint32_t cl, uint32_t cvl, cx<T> m0, cx<T> m1,
-                                             cx<T> m2, cx<T> m3) {
+__device__ __forceinline__ Amps<T, NR> q_u1c(const Amps<T, NR> &a, uint32_t tlc,
+                                             const RegIdx<NR> &ri, uint32_t cl, uint32_t cvl,
+                                             cx<T> m0, cx<T> m1, cx<T> m2, cx<T> m3) {
+  Amps<T, NR> o;
 #pragma unroll
   for (int p = 0; p < NR / 2; ++p) {
     const int j = pair_lo<b>(p), j1 = j | (1 << b);
-    const bool on = (tlc | (ri.rl[j] & cl)) == cvl;
-    const cx<T> a0 = a.v[j], a1 = a.v[j1];
-    a.v[j] = csel(on, mv2(m0, a0, m2, a1), a0);
-    a.v[j1] = csel(on, mv2(m1, a0, m3, a1), a1);
+    const bool on = (tlc | (ri.at(j) & cl)) == cvl;
+    o.v[j] = csel(on, mv2(m0, a.v[j], m2, a.v[j1]), a.v[j]);
+    o.v[j1] = csel(on, mv2(m1, a.v[j], m3, a.v[j1]), a.v[j1]);
   }
-  return a;
+  return o;
 }
 
-template <typename T, int NR, int b>
-__device__ __forceinline__ Amps<T, NR> q_xy(Amps<T, NR> a, uint32_t tlc, const RegIdx<NR> &ri,
-                                            uint32_t cl, uint32_t cvl, bool isy) {
+template <typename T, int NR, int b, bool isy>
+__device__ __forceinline__ Amps<T, NR> q_xy(const Amps<T, NR> &a, uint32_t tlc,
+                                            const RegIdx<NR> &ri, uint32_t cl, uint32_t cvl) {
+  Amps<T, NR> o;
 #pragma unroll
   for (int p = 0; p < NR / 2; ++p) {
     const int j = pair_lo<b>(p), j1 = j | (1 << b);
-    const bool on = (tlc | (ri.rl[j] & cl)) == cvl;
+    const bool on = (tlc | (ri.at(j) & cl)) == cvl;
     const cx<T> a0 = a.v[j], a1 = a.v[j1];
-    const cx<T> n0 = isy ? mk<T>(a1.y, -a1.x) : a1;
-    const cx<T> n1 = isy ? mk<T>(-a0.y, a0.x) : a0;
-    a.v[j] = csel(on, n0, a0);
-    a.v[j1] = csel(on, n1, a1);
+    cx<T> n0 = a1, n1 = a0;
+    if constexpr (isy) {
+      n0 = mk<T>(a1.y, -a1.x);
+      n1 = mk<T>(-a0.y, a0.x);
+    }
+    o.v[j] = csel(on, n0, a0);
+    o.v[j1] = csel(on, n1, a1);
   }
-  return a;
+  return o;
 }
 
+// X on register bit t where register bit c equals cv (c < 0: always): moves only.
+template <typename T, int NR, int t, int c, int cv>
+__device__ __forceinline__ Amps<T, NR> q_xr(const Amps<T, NR> &a) {
+  Amps<T, NR> o;
+#pragma unroll
+  for (int j = 0; j < NR; ++j)
+    o.v[j] = (c < 0 || ((j >> (c < 0 ? 0 : c)) & 1) == cv) ? a.v[j ^ (1 << t)] : a.v[j];
+  return o;
+}
+
+// 4x4 matrix on register bits b0 (matrix bit 0) and b1 (matrix bit 1).
 template <typename T, int NR, int b0, int b1>
-__device__ __forceinline__ Amps<T, NR> q_u2(Amps<T, NR> a, uint32_t tlc, const RegIdx<NR> &ri,
-                                            uint32_t cl, uint32_t cvl,
+__device__ __forceinline__ Amps<T, NR> q_u2(const Amps<T, NR> &a, uint32_t tlc,
+                                            const RegIdx<NR> &ri, uint32_t cl, uint32_t cvl,
                                             const cx<double> *__restrict__ mp) {
   constexpr int lo = b0 < b1 ? b0 : b1, hi = b0 < b1 ? b1 : b0;
-  cx<T> m[16];
+  Amps<T, NR> o;
 #pragma unroll
-  for (int e = 0; e < 16; ++e) m[e] = to_t<T>(mp[e]);
+  for (int r = 0; r < 4; ++r) {
+    const cx<T> c0 = to_t<T>(mp[r]), c1 = to_t<T>(mp[r + 4]);
+    const cx<T> c2 = to_t<T>(mp[r + 8]), c3 = to_t<T>(mp[r + 12]);
 #pragma unroll
-  for (int p = 0; p < NR / 4; ++p) {
-    const int j = pair_lo<hi>(pair_lo<lo>(p));
-    const int i1 = j | (1 << b0), i2 = j | (1 << b1), i3 = j | (1 << b0) | (1 << b1);
-    const bool on = (tlc | (ri.rl[j] & cl)) == cvl;
-    const cx<T> x0 = a.v[j], x1 = a.v[i1], x2 = a.v[i2], x3 = a.v[i3];
-    cx<T> y[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const cx<T> s01 = mv2(m[r], x0, m[r + 4], x1);
-      const cx<T> s23 = mv2(m[r + 8], x2, m[r + 12], x3);
-      y[r] = mk<T>(s01.x + s23.x, s01.y + s23.y);
+    for (int p = 0; p < NR / 4; ++p) {
+      const int j = pair_lo<hi>(pair_lo<lo>(p));
+      const int i1 = j | (1 << b0), i2 = j | (1 << b1), i3 = j | (1 << b0) | (1 << b1);
+      const int dst = r == 0 ? j : r == 1 ? i1 : r == 2 ? i2 : i3;
+      const cx<T> s01 = mv2(c0, a.v[j], c1, a.v[i1]);
+      const cx<T> s23 = mv2(c2, a.v[i2], c3, a.v[i3]);
+      const bool on = (tlc | (ri.at(j) & cl)) == cvl;
+      o.v[dst] = csel(on, mk<T>(s01.x + s23.x, s01.y + s23.y), a.v[dst]);
     }
-    a.v[j] = csel(on, y[0], x0);
-    a.v[i1] = csel(on, y[1], x1);
-    a.v[i2] = csel(on, y[2], x2);
-    a.v[i3] = csel(on, y[3], x3);
   }
-  return a;
+  return o;
+}
+
+// Dispatchers: b < R; the last instantiation is the default so that every
+// path computes the outputs.
+template <int NR>
+constexpr int reg_bits_of() {
+  return NR == 8 ? 3 : NR == 16 ? 4 : 5;
+}
+
+template <typename T, int NR, int B0 = 0>
+__device__ __forceinline__ Amps<T, NR> d_u1(uint32_t b, const Amps<T, NR> &a, cx<T> m0, cx<T> m1,
+                                            cx<T> m2, cx<T> m3) {
+  if constexpr (B0 == reg_bits_of<NR>() - 1) {
+    return q_u1<T, NR, B0>(a, m0, m1, m2, m3);
+  } else {
+    if (b == B0) return q_u1<T, NR, B0>(a, m0, m1, m2, m3);
+    return d_u1<T, NR, B0 + 1>(b, a, m0, m1, m2, m3);
+  }
+}
+
+template <int NR, int B0 = 0>
+__device__ __forceinline__ Amps<float, NR> d_u1_mfma(uint32_t b, const Amps<float, NR> &a,
+                                                     float A0, float A1, float A2, float A3) {
+  if constexpr (B0 == reg_bits_of<NR>() - 1) {
+    return q_u1_mfma<NR, B0>(a, A0, A1, A2, A3);
+  } else {
+    if (b == B0) return q_u1_mfma<NR, B0>(a, A0, A1, A2, A3);
+    return d_u1_mfma<NR, B0 + 1>(b, a, A0, A1, A2, A3);
+  }
+}
+
+template <typename T, int NR, int B0 = 0>
+__device__ __forceinline__ Amps<T, NR> d_u1c(uint32_t b, const Amps<T, NR> &a, uint32_t tlc,
+                                             const RegIdx<NR> &ri, uint32_t cl, uint32_t cvl,
+                                             cx<T> m0, cx<T> m1, cx<T> m2, cx<T> m3) {
+  if constexpr (B0 == reg_bits_of<NR>() - 1) {
+    return q_u1c<T, NR, B0>(a, tlc, ri, cl, cvl, m0, m1, m2, m3);
+  } else {
+    if (b == B0) return q_u1c<T, NR, B0>(a, tlc, ri, cl, cvl, m0, m1, m2, m3);
+    return d_u1c<T, NR, B0 + 1>(b, a, tlc, ri, cl, cvl, m0, m1, m2, m3);
+  }
+}
+
+template <typename T, int NR, int B0 = 0>
+__device__ __forceinline__ Amps<T, NR> d_xy(uint32_t b, const Amps<T, NR> &a, uint32_t tlc,
+                                            const RegIdx<NR> &ri, uint32_t cl, uint32_t cvl,
+                                            bool isy) {
+  if constexpr (B0 == reg_bits_of<NR>() - 1) {
+    if (isy) return q_xy<T, NR, B0, true>(a, tlc, ri, cl, cvl);
+    return q_xy<T, NR, B0, false>(a, tlc, ri, cl, cvl);
+  } else {
+    if (b == B0) {
+      if (isy) return q_xy<T, NR, B0, true>(a, tlc, ri, cl, cvl);
+      return q_xy<T, NR, B0, false>(a, tlc, ri, cl, cvl);
+    }
+    return d_xy<T, NR, B0 + 1>(b, a, tlc, ri, cl, cvl, isy);
+  }
+}
+
+// X on register bit t: control register bit c (C enumerates; c == t or
+// 0xFF: no control), control value cv.
+template <typename T, int NR, int t, int C = 0>
+__device__ __forceinline__ Amps<T, NR> d_xr_c(uint32_t c, uint32_t cv, const Amps<T, NR> &a) {
+  if constexpr (C == reg_bits_of<NR>()) {
+    return q_xr<T, NR, t, -1, 0>(a);
+  } else if constexpr (C == t) {
+    return d_xr_c<T, NR, t, C + 1>(c, cv, a);
+  } else {
+    if (c == C) return cv ? q_xr<T, NR, t, C, 1>(a) : q_xr<T, NR, t, C, 0>(a);
+    return d_xr_c<T, NR, t, C + 1>(c, cv, a);
+  }
+}
+
+template <typename T, int NR, int B0 = 0>
+__device__ __forceinline__ Amps<T, NR> d_xr(uint32_t t, uint32_t c, uint32_t cv,
+                                            const Amps<T, NR> &a) {
+  if constexpr (B0 == reg_bits_of<NR>() - 1) {
+    return d_xr_c<T, NR, B0>(c, cv, a);
+  } else {
+    if (t == B0) return d_xr_c<T, NR, B0>(c, cv, a);
+    return d_xr<T, NR, B0 + 1>(t, c, cv, a);
+  }
+}
+
+// 4x4 on (b0, b1), enumerated as P = b0 * R + b1 over distinct pairs.
+template <typename T, int NR, int P = 1>
+__device__ __forceinline__ Amps<T, NR> d_u2(uint32_t p, const Amps<T, NR> &a, uint32_t tlc,
+                                            const RegIdx<NR> &ri, uint32_t cl, uint32_t cvl,
+                                            const cx<double> *mp) {
+  constexpr int R = reg_bits_of<NR>();
+  constexpr int b0 = P / R, b1 = P % R;
+  constexpr int last = (R - 1) * R + (R - 2);
+  if constexpr (P >= last) {
+    return q_u2<T, NR, R - 1, R - 2>(a, tlc, ri, cl, cvl, mp);
+  } else if constexpr (b0 == b1) {
+    return d_u2<T, NR, P + 1>(p, a, tlc, ri, cl, cvl, mp);
+  } else {
+    if (p == static_cast<uint32_t>(P)) return q_u2<T, NR, b0, b1>(a, tlc, ri, cl, cvl, mp);
+    return d_u2<T, NR, P + 1>(p, a, tlc, ri, cl, cvl, mp);
+  }
 }
 
 // sum_j sign_j * Re(phase * a[j ^ xr] * conj(a[j])), sign_j the parity of
@@ -242,83 +423,10 @@ __device__ __forceinline__ double q_probe(const Amps<T, NR> &a, uint32_t tl,
     const double x1 = a.v[j ^ xr].x, y1 = a.v[j ^ xr].y;
     const double px = pr * x1 - pi * y1, py = pr * y1 + pi * x1;
     double v = px * x0 + py * y0;
-    if (st ^ (__popc(ri.rl[j] & zl) & 1u)) v = -v;
+    if (st ^ (__popc(ri.at(j) & zl) & 1u)) v = -v;
     acc += v;
   }
   return acc;
-}
-
-template <typename T, int NR>
-__device__ __forceinline__ Amps<T, NR> q_u1_dispatch(uint32_t b, Amps<T, NR> a, cx<T> m0,
-                                                     cx<T> m1, cx<T> m2, cx<T> m3) {
-  // b < R; the top register bit is the default so that no path returns `a`.
-  if (b == 0) return q_u1<T, NR, 0>(a, m0, m1, m2, m3);
-  if (b == 1) return q_u1<T, NR, 1>(a, m0, m1, m2, m3);
-  if constexpr (NR == 8) return q_u1<T, NR, 2>(a, m0, m1, m2, m3);
-  if constexpr (NR >= 16) if (b == 2) return q_u1<T, NR, 2>(a, m0, m1, m2, m3);
-  if constexpr (NR == 16) return q_u1<T, NR, 3>(a, m0, m1, m2, m3);
-  if constexpr (NR == 32) {
-    if (b == 3) return q_u1<T, NR, 3>(a, m0, m1, m2, m3);
-    return q_u1<T, NR, 4>(a, m0, m1, m2, m3);
-  }
-}
-
-template <typename T, int NR>
-__device__ __forceinline__ Amps<T, NR> q_u1c_dispatch(uint32_t b, Amps<T, NR> a, uint32_t tlc,
-                                                      const RegIdx<NR> &ri, uint32_t cl,
-                                                      uint32_t cvl, cx<T> m0, cx<T> m1,
-                                                      cx<T> m2, cx<T> m3) {
-  if (b == 0) return q_u1c<T, NR, 0>(a, tlc, ri, cl, cvl, m0, m1, m2, m3);
-  if (b == 1) return q_u1c<T, NR, 1>(a, tlc, ri, cl, cvl, m0, m1, m2, m3);
-  if constexpr (NR == 8) return q_u1c<T, NR, 2>(a, tlc, ri, cl, cvl, m0, m1, m2, m3);
-  if constexpr (NR >= 16) if (b == 2) return q_u1c<T, NR, 2>(a, tlc, ri, cl, cvl, m0, m1, m2, m3);
-  if constexpr (NR == 16) return q_u1c<T, NR, 3>(a, tlc, ri, cl, cvl, m0, m1, m2, m3);
-  if constexpr (NR == 32) {
-    if (b == 3) return q_u1c<T, NR, 3>(a, tlc, ri, cl, cvl, m0, m1, m2, m3);
-    return q_u1c<T, NR, 4>(a, tlc, ri, cl, cvl, m0, m1, m2, m3);
-  }
-}
-
-template <typename T, int NR>
-__device__ __forceinline__ Amps<T, NR> q_xy_dispatch(uint32_t b, Amps<T, NR> a, uint32_t tlc,
-                                                     const RegIdx<NR> &ri, uint32_t cl,
-                                                     uint32_t cvl, bool isy) {
-  if (b == 0) return q_xy<T, NR, 0>(a, tlc, ri, cl, cvl, isy);
-  if (b == 1) return q_xy<T, NR, 1>(a, tlc, ri, cl, cvl, isy);
-  if constexpr (NR == 8) return q_xy<T, NR, 2>(a, tlc, ri, cl, cvl, isy);
-  if constexpr (NR >= 16) if (b == 2) return q_xy<T, NR, 2>(a, tlc, ri, cl, cvl, isy);
-  if constexpr (NR == 16) return q_xy<T, NR, 3>(a, tlc, ri, cl, cvl, isy);
-  if constexpr (NR == 32) {
-    if (b == 3) return q_xy<T, NR, 3>(a, tlc, ri, cl, cvl, isy);
-    return q_xy<T, NR, 4>(a, tlc, ri, cl, cvl, isy);
-  }
-}
-
-// Pair (b0, b1) of distinct register bits, enumerated as P = b0 * R + b1; the
-// last valid pair is the default so that no path returns `a`.
-template <typename T, int NR, int R, int P>
-__device__ __forceinline__ Amps<T, NR> q_u2_pick(uint32_t p, Amps<T, NR> a, uint32_t tlc,
-                                                 const RegIdx<NR> &ri, uint32_t cl, uint32_t cvl,
-                                                 const cx<double> *mp) {
-  constexpr int b0 = P / R, b1 = P % R;
-  constexpr int last = (R - 1) * R + (R - 2);
-  if constexpr (P >= last) {
-    return q_u2<T, NR, R - 1, R - 2>(a, tlc, ri, cl, cvl, mp);
-  } else if constexpr (b0 == b1) {
-    return q_u2_pick<T, NR, R, P + 1>(p, a, tlc, ri, cl, cvl, mp);
-  } else {
-    if (p == P) return q_u2<T, NR, b0, b1>(a, tlc, ri, cl, cvl, mp);
-    return q_u2_pick<T, NR, R, P + 1>(p, a, tlc, ri, cl, cvl, mp);
-  }
-}
-
-template <typename T, int NR>
-__device__ __forceinline__ Amps<T, NR> q_u2_dispatch(uint32_t b0, uint32_t b1, Amps<T, NR> a,
-                                                     uint32_t tlc, const RegIdx<NR> &ri,
-                                                     uint32_t cl, uint32_t cvl,
-                                                     const cx<double> *mp) {
-  constexpr int R = NR == 8 ? 3 : NR == 16 ? 4 : 5;
-  return q_u2_pick<T, NR, R, 1>(b0 * R + b1, a, tlc, ri, cl, cvl, mp);
 }
 
 template <typename T, int NR, int X>
@@ -349,88 +457,106 @@ __device__ __forceinline__ LayoutState<1 << R> make_state(const RLayout &lay, ui
 }
 
 // A register op that rewrites every amplitude (a layout change or a gate):
-// returns the new amplitudes and never passes the input through unchanged,
-// so that the op loop can alternate between two register banks without
-// copies.  Controls fixed by the tile are folded into the local predicate
-// (an unmet one turns it into "never"); a 2x2 matrix becomes the identity.
+// returns the new amplitudes and never passes the input through unchanged
+// (except probes and Q_NOP), so that the op loop can alternate between two
+// register banks without copies.  Controls fixed by the tile are folded into
+// the local predicate (an unmet one turns it into "never"; Q_U1 carries no
+// controls).
+template <typename T>
+__device__ __forceinline__ cx<T> mat_at(const T *m, int e) {
+  return mk<T>(m[2 * e], m[2 * e + 1]);
+}
+
 template <typename T, int B, int R>
 __device__ __forceinline__ Amps<T, 1 << R> rw_op(const ROp<T> &op, const Amps<T, 1 << R> &a,
                                                  LayoutState<1 << R> &st, cx<T> *lds,
                                                  double *red, uint64_t base, uint32_t tid,
                                                  const TileOp *__restrict__ tops,
                                                  const cx<double> *__restrict__ pool,
-                                                 double *__restrict__ probes) {
+                                                 const ProbeSink &sink) {
   constexpr int NR = 1 << R;
   constexpr int TB = B - R;
-  const uint32_t kind = op.kind;
+  const uint32_t kind = op.kind, ra = op.a, rb = op.b, extra = op.extra;
   if (kind == Q_PROBE || kind == Q_NOP) {
     if (kind == Q_PROBE) {
       const RProbe &pr = op.u.pr;
       const uint32_t sgb = __popcll(base & pr.zg) & 1u;
       double acc = q_probe_dispatch<T, NR, 0>(pr.xr, a, st.tl, st.ri, pr.zl, sgb, pr.pr, pr.pi);
       acc = block_sum_nt<(1 << TB)>(acc, red);
-      if (tid == 0) probes[static_cast<uint64_t>(op.extra) * gridDim.x + blockIdx.x] = acc;
+      if (tid == 0) sink.put(extra, acc);
     }
     return a;
   }
   if (kind == Q_RELAYOUT) {
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < NR; ++j) lds[st.tl | st.ri.rl[j]] = a.v[j];
+    for (int j = 0; j < NR; ++j) lds[st.tl | st.ri.at(j)] = a.v[j];
     __syncthreads();
     st = make_state<R, TB>(op.u.lay, tid);
     Amps<T, NR> o;
 #pragma unroll
-    for (int j = 0; j < NR; ++j) o.v[j] = lds[st.tl | st.ri.rl[j]];
+    for (int j = 0; j < NR; ++j) o.v[j] = lds[st.tl | st.ri.at(j)];
     return o;
   }
-  // Controls fixed by the tile fold into the local predicate: an unmet one
-  // makes it "never" (Q_U1 carries no controls; the host emits Q_U1C then).
   const bool tile_ok = (base & op.cg) == op.cvg;
   const uint32_t cl = tile_ok ? op.cl : 0u;
   const uint32_t cvl = tile_ok ? op.cvl : 1u;
   const uint32_t tlc = st.tl & cl;
   if (kind == Q_U1) {
-    const cx<T> m0 = mk<T>(op.u.m[0], op.u.m[1]), m1 = mk<T>(op.u.m[2], op.u.m[3]);
-    const cx<T> m2 = mk<T>(op.u.m[4], op.u.m[5]), m3 = mk<T>(op.u.m[6], op.u.m[7]);
-    return q_u1_dispatch<T, NR>(op.a, a, m0, m1, m2, m3);
+    if constexpr (sizeof(T) == 4) {
+      // A operand of issue k: row (lane % 4), column k of the real 4x4 matrix
+      const uint32_t r = threadIdx.x & 3u;
+      const bool r0 = r == 0, r1 = r == 1, r2 = r == 2;
+      const float *mr = op.u.mr;
+      const float A0 = r0 ? mr[0] : r1 ? mr[4] : r2 ? mr[8] : mr[12];
+      const float A1 = r0 ? mr[1] : r1 ? mr[5] : r2 ? mr[9] : mr[13];
+      const float A2 = r0 ? mr[2] : r1 ? mr[6] : r2 ? mr[10] : mr[14];
+      const float A3 = r0 ? mr[3] : r1 ? mr[7] : r2 ? mr[11] : mr[15];
+      return d_u1_mfma<NR>(ra, a, A0, A1, A2, A3);
+    } else {
+      const cx<T> m0 = mat_at<T>(op.u.m, 0), m1 = mat_at<T>(op.u.m, 1), m2 = mat_at<T>(op.u.m, 2), m3 = mat_at<T>(op.u.m, 3);
+      return d_u1<T, NR>(ra, a, m0, m1, m2, m3);
+    }
   }
+  if (kind == Q_XR) return d_xr<T, NR>(ra, rb, extra, a);
   if (kind == Q_U1C) {
-    const cx<T> m0 = mk<T>(op.u.m[0], op.u.m[1]), m1 = mk<T>(op.u.m[2], op.u.m[3]);
-    const cx<T> m2 = mk<T>(op.u.m[4], op.u.m[5]), m3 = mk<T>(op.u.m[6], op.u.m[7]);
-    return q_u1c_dispatch<T, NR>(op.a, a, tlc, st.ri, cl, cvl, m0, m1, m2, m3);
+    const cx<T> m0 = mat_at<T>(op.u.m, 0), m1 = mat_at<T>(op.u.m, 1), m2 = mat_at<T>(op.u.m, 2), m3 = mat_at<T>(op.u.m, 3);
+    return d_u1c<T, NR>(ra, a, tlc, st.ri, cl, cvl, m0, m1, m2, m3);
   }
-  if (kind == Q_X || kind == Q_Y) return q_xy_dispatch<T, NR>(op.a, a, tlc, st.ri, cl, cvl, kind == Q_Y);
-  if (kind == Q_U2) return q_u2_dispatch<T, NR>(op.a, op.b, a, tlc, st.ri, cl, cvl, pool + op.extra);
+  if (kind == Q_X || kind == Q_Y) {
+    const bool isy = kind == Q_Y;
+    return d_xy<T, NR>(ra, a, tlc, st.ri, cl, cvl, isy);
+  }
+  if (kind == Q_U2) return d_u2<T, NR>(ra * reg_bits_of<NR>() + rb, a, tlc, st.ri, cl, cvl, pool + extra);
   const cx<T> one = mk<T>(1, 0);
   Amps<T, NR> o;
   if (kind == Q_D1) {
-    const cx<T> d0 = mk<T>(op.u.m[0], op.u.m[1]), d1 = mk<T>(op.u.m[2], op.u.m[3]);
-    const uint32_t t = op.a;
+    const cx<T> d0 = mat_at<T>(op.u.m, 0), d1 = mat_at<T>(op.u.m, 1);
+    const uint32_t t = ra;
 #pragma unroll
     for (int j = 0; j < NR; ++j) {
-      const uint32_t l = st.tl | st.ri.rl[j];
+      const uint32_t l = st.tl | st.ri.at(j);
       const cx<T> d = ((l & cl) == cvl) ? csel(((l >> t) & 1u) != 0, d1, d0) : one;
       o.v[j] = cmul(a.v[j], d);
     }
     return o;
   }
   if (kind == Q_PHASE) {
-    const cx<T> ph = mk<T>(op.u.m[0], op.u.m[1]);
+    const cx<T> ph = mat_at<T>(op.u.m, 0);
 #pragma unroll
     for (int j = 0; j < NR; ++j)
-      o.v[j] = cmul(a.v[j], csel((tlc | (st.ri.rl[j] & cl)) == cvl, ph, one));
+      o.v[j] = cmul(a.v[j], csel((tlc | (st.ri.at(j) & cl)) == cvl, ph, one));
     return o;
   }
   // Q_DIAG (the last kind, so that every path computes new amplitudes)
-  const TileOp &top = tops[op.extra];
+  const TileOp &top = tops[extra];
   const uint32_t k = top.k;
   uint32_t ivb = 0;
   for (uint32_t d = 0; d < k; ++d)
     if (top.pos[d] == kNotLocal) ivb |= static_cast<uint32_t>((base >> top.gpos[d]) & 1ull) << d;
 #pragma unroll
   for (int j = 0; j < NR; ++j) {
-    const uint32_t l = st.tl | st.ri.rl[j];
+    const uint32_t l = st.tl | st.ri.at(j);
     uint32_t iv = ivb;
     for (uint32_t d = 0; d < k; ++d)
       if (top.pos[d] != kNotLocal) iv |= ((l >> top.pos[d]) & 1u) << d;
@@ -439,29 +565,24 @@ __device__ __forceinline__ Amps<T, 1 << R> rw_op(const ROp<T> &op, const Amps<T,
   return o;
 }
 
-// One tile pass: ops[pd->op_begin .. + pd->nops) are ROps; TileOp primitives
-// (Q_LDS, Q_DIAG) index `tops`.  Probe partials go to probes[slot * ntiles +
-// tile].  The op list has the form
+// One tile of a register-layout pass: ops[pd->op_begin .. + pd->nops) are
+// ROps; TileOp primitives (Q_LDS, Q_DIAG) index `tops`.  The op list has the
+// form
 //     LOAD  { register ops }  ( SPILL { LDS ops } FILL { register ops } )*  STORE
 // where each register phase holds an even number of ops (Q_NOP pads).
-template <typename T, int B, int R>
-__global__ __launch_bounds__(1 << (B - R)) void k_reg_pass(
-    cx<T> *__restrict__ sv, const PassDesc *__restrict__ pd, const ROp<T> *__restrict__ rops,
-    const TileOp *__restrict__ tops, const cx<double> *__restrict__ pool,
-    double *__restrict__ probes) {
+template <typename T, int B, int R, int MATCAP>
+__device__ __forceinline__ void reg_tile(cx<T> *__restrict__ sv, const PassDesc *__restrict__ pd,
+                                         const ROp<T> *__restrict__ rops,
+                                         const TileOp *__restrict__ tops,
+                                         const cx<double> *__restrict__ pool,
+                                         const ProbeSink &sink, uint64_t tile_id, cx<T> *lds,
+                                         cx<T> *mat, uint32_t *doff, double *red) {
   constexpr int TB = B - R;
   constexpr int NT = 1 << TB;
   constexpr int NR = 1 << R;
   constexpr uint32_t N = 1u << B;
-  __shared__ cx<T> lds[N];
-  __shared__ cx<T> mat[16];
-  __shared__ uint32_t doff[32];
-  __shared__ double red[NT / 64];
-
   const uint32_t tid = threadIdx.x;
-  // HBM offset of the tile: the workgroup index spread over the qubits
-  // outside the tile (zeros inserted at the tile qubits, ascending).
-  uint64_t base = blockIdx.x;
+  uint64_t base = tile_id;
 #pragma unroll
   for (int i = 0; i < B; ++i) {
     const uint32_t q = pd->spos[i];
@@ -483,34 +604,51 @@ __global__ __launch_bounds__(1 << (B - R)) void k_reg_pass(
       for (int j = 0; j < NR; ++j) a.v[j] = p[ro.gr[j]];
     } else {
 #pragma unroll
-      for (int j = 0; j < NR; ++j) a.v[j] = lds[st.tl | st.ri.rl[j]];
+      for (int j = 0; j < NR; ++j) a.v[j] = lds[st.tl | st.ri.at(j)];
     }
     // Register ops alternate between banks a and b.
     const uint32_t cnt = e.extra;
     for (uint32_t i = 0; i < cnt; i += 2) {
-      const Amps<T, NR> b = rw_op<T, B, R>(opl[o], a, st, lds, red, base, tid, tops, pool, probes);
-      a = rw_op<T, B, R>(opl[o + 1], b, st, lds, red, base, tid, tops, pool, probes);
+      const Amps<T, NR> b = rw_op<T, B, R>(opl[o], a, st, lds, red, base, tid, tops, pool, sink);
+      a = rw_op<T, B, R>(opl[o + 1], b, st, lds, red, base, tid, tops, pool, sink);
       o += 2;
     }
     const ROp<T> &l = opl[o++];
     if (l.kind == Q_STORE) {
+      if (l.extra) return;  // read-only pass
       const RegOff<NR> ro = reg_offset<R>(l.u.lay);
       cx<T> *p = sv + base + thread_offset<TB>(l.u.lay, tid);
 #pragma unroll
       for (int j = 0; j < NR; ++j) p[ro.gr[j]] = a.v[j];
-      break;
+      return;
     }
     // SPILL: the tile goes to LDS in natural order for the LDS primitives.
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < NR; ++j) lds[st.tl | st.ri.rl[j]] = a.v[j];
+    for (int j = 0; j < NR; ++j) lds[st.tl | st.ri.at(j)] = a.v[j];
     while (opl[o].kind == Q_LDS) {
-      __syncthreads();
-      lds_apply<T, NT, B, 16>(tops[opl[o].extra], lds, mat, doff, red, N, base, pool, probes,
-                              gridDim.x);
+      const TileOp &top = tops[opl[o].extra];
+      if ((base & top.cg) == top.cvg) {
+        __syncthreads();
+        lds_apply<T, NT, B, MATCAP>(top, lds, mat, doff, red, N, base, pool, sink);
+      }
       ++o;
     }
   }
+}
+
+// One workgroup per tile.
+template <typename T, int B, int R>
+__global__ __launch_bounds__(1 << (B - R)) void k_reg_pass(
+    cx<T> *__restrict__ sv, const PassDesc *__restrict__ pd, const ROp<T> *__restrict__ rops,
+    const TileOp *__restrict__ tops, const cx<double> *__restrict__ pool,
+    double *__restrict__ probes) {
+  __shared__ cx<T> lds[1 << B];
+  __shared__ cx<T> mat[16];
+  __shared__ uint32_t doff[32];
+  __shared__ double red[(1 << (B - R)) / 64];
+  const ProbeSink sink{probes, blockIdx.x, gridDim.x, false};
+  reg_tile<T, B, R, 16>(sv, pd, rops, tops, pool, sink, blockIdx.x, lds, mat, doff, red);
 }
 
 }  // namespace aerhip

@@ -199,6 +199,12 @@ class QubitVector:
         L.check(self._L.aerhip_norm(self._h, C.byref(out)))
         return out.value
 
+    def inner_product(self, other: "QubitVector") -> complex:
+        """<self|other> accumulated in FP64 (the states may differ in precision)."""
+        re, im = C.c_double(), C.c_double()
+        L.check(self._L.aerhip_qv_inner_product(self._h, other._h, C.byref(re), C.byref(im)))
+        return complex(re.value, im.value)
+
     def probability(self, outcome: int) -> float:
         out = C.c_double()
         L.check(self._L.aerhip_probability(self._h, int(outcome), C.byref(out)))

@@ -293,3 +293,41 @@ def test_register_layouts_match_oracle(precision, reg_bits):
     close(e, o, precision)
     if reg_bits:
         assert e.stats()["gathers"] > 0
+
+
+@pytest.mark.parametrize("pa,pb", [("double", "double"), ("single", "double"), ("double", "single")])
+def test_inner_product(pa, pb):
+    """<a|b> on the device equals numpy's vdot of the copied-out states."""
+    n = 14
+    print(f"case: inner product {pa}/{pb}", flush=True)
+    rng = np.random.default_rng(11)
+    u, v = D.random_state(n, rng), D.random_state(n, rng)
+    a, b = engine(n, pa), engine(n, pb)
+    a.initialize_from_vector(u)
+    b.initialize_from_vector(v)
+    got = a.inner_product(b)
+    want = np.vdot(a.copy_to_vector(), b.copy_to_vector())
+    assert abs(got - want) < 1e-12
+    assert abs(a.inner_product(a) - 1) < (1e-12 if pa == "double" else 1e-6)
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_pauli_strings_wider_than_a_tile(precision):
+    """apply_pauli and saved expectation values with X/Y on more qubits than a
+    tile holds (split into a sign/phase part and single-qubit X; probe sweeps)."""
+    n = 15
+    print(f"case: wide Pauli strings {precision}", flush=True)
+    e, o, rng = pair(n, 12, precision)
+    qs = list(range(n))
+    lab = "".join(rng.choice(list("XY"), n - 1)) + "Z"
+    e.apply_pauli(qs, lab, 0.6 + 0.8j)
+    o.apply_pauli(qs, lab, 0.6 + 0.8j)
+    close(e, o, precision)
+    labels = ["".join(rng.choice(list("XYZ"), n)) for _ in range(3)] + ["Y" * n, "I" * (n - 1) + "X"]
+    for lab in labels:
+        e.save_expval_pauli(qs, lab, 0.3 - 0.4j)
+    m = D.random_unitary(2, rng)
+    e.apply_matrix([2], m)
+    o2 = [o.expval_pauli(qs, lab, 0.3 - 0.4j) for lab in labels]
+    got = e.saved_expvals()
+    assert np.max(np.abs(np.array(got) - np.array(o2))) < (1e-12 if precision == "double" else 1e-5)
