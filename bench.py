@@ -26,6 +26,12 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
 
+# HBM bytes per tile-pass launch of the default workload (30 qubits, c128),
+# from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (profiles/reg_pass_30q_brickwork_c128.md):
+# 2.0 x 8 388 816 KiB fetched (FETCH_SIZE counts half the bytes, calibrated on a
+# kernel that reads the 16 GiB state once) + 16 777 216 KiB written.
+MEASURED_TRAFFIC_30Q_C128 = (2 * 8388816 + 16777216) * 1024
+
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
@@ -157,7 +163,10 @@ def run_mixed_precision(n: int, layers: int, seed: int) -> dict:
         zz[prec] = qv.expval_pauli_multi(list(range(n)), labels)
         states[prec] = qv
     ov = states["single"].inner_product(states["double"])
-    out["one_minus_fidelity"] = 1.0 - abs(ov) ** 2
+    n64 = states["single"].norm()
+    n128 = states["double"].norm()
+    out["one_minus_fidelity"] = 1.0 - abs(ov) ** 2 / (n64 * n128)
+    out["norm_c64"] = n64
     out["mean_abs_zz_error"] = float(np.mean(np.abs(zz["single"] - zz["double"])))
     out["c64_over_c128"] = out["single"]["gates_per_s"] / out["double"]["gates_per_s"]
     for qv in states.values():
@@ -333,7 +342,8 @@ def main() -> int:
         "passes_per_step": c128["passes_per_step"],
         "roofline": {"bound": "hbm", "achieved": c128["kernel_gbs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": c128["kernel_gbs"] / HBM_PEAK_GBS,
-                     "traffic": None},
+                     "traffic": MEASURED_TRAFFIC_30Q_C128 if (n, args.layers) == (30, 40) else None,
+                     "algorithmic_bytes_per_launch": c128["sweep_bytes"]},
         "cpu_baseline": cpu,
         "extra": extra,
     }

@@ -76,8 +76,8 @@ template <typename T>
 struct alignas(128) ROp {
   uint32_t kind;
   uint32_t a, b;
-  uint32_t cl, cvl;  // local control mask and value
   uint32_t extra;    // probe slot, TileOp index or pool offset
+  uint32_t cl, cvl;  // local control mask and value
   uint64_t cg, cvg;  // controls on qubits fixed by the tile
   uint32_t pad[6];
   union {
@@ -224,17 +224,35 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 template <int NR, int b>
 __device__ __forceinline__ Amps<float, NR> q_u1_mfma(const Amps<float, NR> &a, float A0, float A1,
                                                      float A2, float A3) {
+  // Issue k of every pair before issue k+1 of any: the accumulation chains
+  // interleave, so no issue waits on the previous one's result.
+  f32x4 acc[NR / 2];
+#pragma unroll
+  for (int p = 0; p < NR / 2; ++p) {
+    const int j = pair_lo<b>(p);
+    acc[p] = __builtin_amdgcn_mfma_f32_4x4x1f32(A0, a.v[j].x, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+  }
+#pragma unroll
+  for (int p = 0; p < NR / 2; ++p) {
+    const int j = pair_lo<b>(p);
+    acc[p] = __builtin_amdgcn_mfma_f32_4x4x1f32(A1, a.v[j].y, acc[p], 0, 0, 0);
+  }
+#pragma unroll
+  for (int p = 0; p < NR / 2; ++p) {
+    const int j1 = pair_lo<b>(p) | (1 << b);
+    acc[p] = __builtin_amdgcn_mfma_f32_4x4x1f32(A2, a.v[j1].x, acc[p], 0, 0, 0);
+  }
+#pragma unroll
+  for (int p = 0; p < NR / 2; ++p) {
+    const int j1 = pair_lo<b>(p) | (1 << b);
+    acc[p] = __builtin_amdgcn_mfma_f32_4x4x1f32(A3, a.v[j1].y, acc[p], 0, 0, 0);
+  }
   Amps<float, NR> o;
 #pragma unroll
   for (int p = 0; p < NR / 2; ++p) {
     const int j = pair_lo<b>(p), j1 = j | (1 << b);
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    acc = __builtin_amdgcn_mfma_f32_4x4x1f32(A0, a.v[j].x, acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_4x4x1f32(A1, a.v[j].y, acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_4x4x1f32(A2, a.v[j1].x, acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_4x4x1f32(A3, a.v[j1].y, acc, 0, 0, 0);
-    o.v[j] = mk<float>(acc[0], acc[1]);
-    o.v[j1] = mk<float>(acc[2], acc[3]);
+    o.v[j] = mk<float>(acc[p][0], acc[p][1]);
+    o.v[j1] = mk<float>(acc[p][2], acc[p][3]);
   }
   return o;
 }
@@ -456,6 +474,16 @@ __device__ __forceinline__ LayoutState<1 << R> make_state(const RLayout &lay, ui
   return s;
 }
 
+// The first 16 bytes of an op record (kind, register bits, extra), read ahead.
+struct OpHead {
+  uint32_t kind, a, b, extra;
+};
+
+template <typename T>
+__device__ __forceinline__ OpHead load_head(const ROp<T> &op) {
+  return OpHead{op.kind, op.a, op.b, op.extra};
+}
+
 // A register op that rewrites every amplitude (a layout change or a gate):
 // returns the new amplitudes and never passes the input through unchanged
 // (except probes and Q_NOP), so that the op loop can alternate between two
@@ -468,7 +496,8 @@ __device__ __forceinline__ cx<T> mat_at(const T *m, int e) {
 }
 
 template <typename T, int B, int R>
-__device__ __forceinline__ Amps<T, 1 << R> rw_op(const ROp<T> &op, const Amps<T, 1 << R> &a,
+__device__ __forceinline__ Amps<T, 1 << R> rw_op(const ROp<T> &op, const OpHead &hd,
+                                                 const Amps<T, 1 << R> &a,
                                                  LayoutState<1 << R> &st, cx<T> *lds,
                                                  double *red, uint64_t base, uint32_t tid,
                                                  const TileOp *__restrict__ tops,
@@ -476,7 +505,7 @@ __device__ __forceinline__ Amps<T, 1 << R> rw_op(const ROp<T> &op, const Amps<T,
                                                  const ProbeSink &sink) {
   constexpr int NR = 1 << R;
   constexpr int TB = B - R;
-  const uint32_t kind = op.kind, ra = op.a, rb = op.b, extra = op.extra;
+  const uint32_t kind = hd.kind, ra = hd.a, rb = hd.b, extra = hd.extra;
   if (kind == Q_PROBE || kind == Q_NOP) {
     if (kind == Q_PROBE) {
       const RProbe &pr = op.u.pr;
@@ -606,11 +635,15 @@ __device__ __forceinline__ void reg_tile(cx<T> *__restrict__ sv, const PassDesc 
 #pragma unroll
       for (int j = 0; j < NR; ++j) a.v[j] = lds[st.tl | st.ri.at(j)];
     }
-    // Register ops alternate between banks a and b.
+    // Register ops alternate between banks a and b.  The head of the op
+    // after the current one is loaded while the current op runs.
     const uint32_t cnt = e.extra;
+    OpHead h0 = load_head(opl[o]);
     for (uint32_t i = 0; i < cnt; i += 2) {
-      const Amps<T, NR> b = rw_op<T, B, R>(opl[o], a, st, lds, red, base, tid, tops, pool, sink);
-      a = rw_op<T, B, R>(opl[o + 1], b, st, lds, red, base, tid, tops, pool, sink);
+      const OpHead h1 = load_head(opl[o + 1]);
+      const Amps<T, NR> b = rw_op<T, B, R>(opl[o], h0, a, st, lds, red, base, tid, tops, pool, sink);
+      h0 = load_head(opl[o + 2]);
+      a = rw_op<T, B, R>(opl[o + 1], h1, b, st, lds, red, base, tid, tops, pool, sink);
       o += 2;
     }
     const ROp<T> &l = opl[o++];
