@@ -24,7 +24,8 @@ struct BatchCircuit {
 };
 
 template <typename T, int B, int R>
-__global__ __launch_bounds__(kThreads) void k_batch(cx<T> *__restrict__ states,
+__global__ __launch_bounds__(kThreads)
+__attribute__((amdgpu_waves_per_eu(reg_pass_waves<T, B, R>()))) void k_batch(cx<T> *__restrict__ states,
                                                     const BatchCircuit *__restrict__ circ,
                                                     const PassDesc *__restrict__ passes,
                                                     const ROp<T> *__restrict__ rops,

@@ -740,23 +740,39 @@ void validate_register_program(const ROp<double> *p, size_t n, uint32_t B, uint3
     if (o >= n) fail("missing phase");
     const ROp<double> &e = p[o++];
     if (e.kind != Q_LOAD && e.kind != Q_FILL) fail("phase must start with LOAD or FILL");
+    uint32_t regmask = 0;  // tile bits of the register set
+    for (uint32_t r = 0; r < R; ++r) regmask |= 1u << byte_at(e.u.lay.rp, static_cast<int>(r));
+    auto check_map = [&](const ROp<double> &r) {
+      if (!(r.kind & F_MAP)) return;
+      uint32_t span = 0;
+      for (uint32_t q = 0; q < R; ++q) {
+        if (r.map_bm[q] & ~regmask) fail("relabelling outside the register bits");
+        span |= r.map_bm[q];
+      }
+      if (r.map_t0 & ~regmask) fail("relabelling translation outside the register bits");
+      (void)span;
+    };
     if ((o == 1) != (e.kind == Q_LOAD)) fail("LOAD must come first, and only there");
     check_layout(e.u.lay);
     if (e.extra & 1u) fail("odd phase length");
     if (o + e.extra >= n) fail("phase overruns the program");
     for (uint32_t i = 0; i < e.extra; ++i) {
       const ROp<double> &r = p[o++];
-      switch (r.kind) {
-        case Q_RELAYOUT: check_layout(r.u.lay); break;
+      check_map(r);
+      switch (r.kind & ~F_MAP) {
+        case Q_RELAYOUT:
+          check_layout(r.u.lay);
+          regmask = 0;
+          for (uint32_t q = 0; q < R; ++q) regmask |= 1u << byte_at(r.u.lay.rp, static_cast<int>(q));
+          break;
+        case Q_U1M:
+          if (r.a == 0 || r.a >= (1u << R) || r.extra >= (1u << ((1u << R) / 2))) fail("relabelled 2x2 gate");
+          break;
         case Q_U1: case Q_U1C: case Q_X: case Q_Y:
           if (r.a >= R) fail("register bit of a gate");
           break;
         case Q_U2:
           if (r.a >= R || r.b >= R || r.a == r.b) fail("register bits of a 2-qubit gate");
-          break;
-        case Q_XR:
-          if (r.a >= R || (r.b != 0xFFu && (r.b >= R || r.b == r.a)) || r.extra > 1)
-            fail("register-controlled X");
           break;
         case Q_D1: if (r.a >= B) fail("diagonal bit"); break;
         case Q_DIAG: if (r.extra >= ntops) fail("diagonal index"); break;
@@ -766,12 +782,13 @@ void validate_register_program(const ROp<double> *p, size_t n, uint32_t B, uint3
       }
     }
     const ROp<double> &l = p[o++];
-    if (l.kind == Q_STORE) {
+    check_map(l);
+    if ((l.kind & ~F_MAP) == Q_STORE) {
       check_layout(l.u.lay);
       if (o != n) fail("ops after STORE");
       return;
     }
-    if (l.kind != Q_SPILL) fail("phase must end with STORE or SPILL");
+    if ((l.kind & ~F_MAP) != Q_SPILL) fail("phase must end with STORE or SPILL");
     while (o < n && p[o].kind == Q_LDS) {
       if (p[o].extra >= ntops) fail("LDS op index");
       ++o;
@@ -849,6 +866,55 @@ void register_program(const std::vector<TileOp> &in, uint32_t B, uint32_t R, uin
   // ops that follow), an even number of register ops (Q_NOP pads), then
   // LEAVE (STORE/SPILL).
   size_t enter_at = 0;
+  // Register relabelling (see RegIdx in reg_pass.hpp): physical register j
+  // holds the logical register index lin(j) ^ cinv, lin(j) = XOR of colinv[s]
+  // over the set bits s of j; logical register bit r is tile bit rp[r] of the
+  // current layout.  X and CX between register qubits only change the map.
+  uint32_t colinv[8], cinv = 0, cur_rs = 0;
+  bool map_dirty = false;
+  auto map_reset = [&] {
+    for (uint32_t q = 0; q < 8; ++q) colinv[q] = 1u << q;
+    cinv = 0;
+    map_dirty = false;
+  };
+  map_reset();
+  auto tile_mask = [&](uint32_t logical) {  // logical register mask -> tile bits
+    uint32_t t = 0;
+    uint32_t r = 0;
+    for (uint32_t b = 0; b < B; ++b)
+      if (cur_rs & (1u << b)) {
+        if (logical & (1u << r)) t |= 1u << b;
+        ++r;
+      }
+    return t;
+  };
+  auto lin = [&](uint32_t x) {
+    uint32_t y = 0;
+    for (uint32_t q = 0; q < R; ++q)
+      if (x & (1u << q)) y ^= colinv[q];
+    return y;
+  };
+  auto phys_of = [&](uint32_t y) {  // the physical mask x with lin(x) == y
+    for (uint32_t x = 0; x < (1u << R); ++x)
+      if (lin(x) == y) return x;
+    throw std::logic_error("aerhip: register relabelling is not invertible");
+  };
+  auto row_of = [&](uint32_t b) {  // physical bits whose logical image has bit b
+    uint32_t f = 0;
+    for (uint32_t q = 0; q < R; ++q)
+      if (colinv[q] & (1u << b)) f |= 1u << q;
+    return f;
+  };
+  auto identity_on = [&](uint32_t b) {
+    return phys_of(1u << b) == (1u << b) && row_of(b) == (1u << b) && !((cinv >> b) & 1u);
+  };
+  auto stamp = [&](ROp<double> &r) {
+    if (!map_dirty) return;
+    r.kind |= F_MAP;
+    for (uint32_t q = 0; q < R; ++q) r.map_bm[q] = static_cast<uint16_t>(tile_mask(colinv[q]));
+    r.map_t0 = static_cast<uint16_t>(tile_mask(cinv));
+    map_dirty = false;
+  };
   auto emit = [&](uint32_t kind, uint32_t rs) {
     if (kind == Q_STORE || kind == Q_SPILL) {
       uint32_t cnt = static_cast<uint32_t>(rops.size() - enter_at - 1);
@@ -866,10 +932,24 @@ void register_program(const std::vector<TileOp> &in, uint32_t B, uint32_t R, uin
     std::memset(&r, 0, sizeof(r));
     r.kind = kind;
     r.u.lay = make_layout(rs, B, spos);
+    if (kind == Q_STORE || kind == Q_SPILL || kind == Q_RELAYOUT) stamp(r);
     rops.push_back(r);
+    if (kind == Q_LOAD || kind == Q_FILL || kind == Q_RELAYOUT) {
+      map_reset();
+      cur_rs = rs;
+    }
   };
   auto regidx = [](uint32_t rs, uint32_t b) {
     return static_cast<uint32_t>(__builtin_popcount(rs & ((1u << b) - 1u)));
+  };
+  // Ops that need the registers in the layout's own order (pair structure on
+  // their register bits) start from a fresh layout when relabelled.
+  auto need_identity = [&](uint32_t rs, std::initializer_list<uint32_t> bits) {
+    bool ok = true;
+    for (uint32_t b : bits) ok = ok && identity_on(b);
+    if (ok) return;
+    emit(Q_RELAYOUT, rs);
+    if (relayouts) ++*relayouts;
   };
   auto emit_gate = [&](const TileOp &op, uint32_t rs) {
     ROp<double> r;
@@ -879,32 +959,66 @@ void register_program(const std::vector<TileOp> &in, uint32_t B, uint32_t R, uin
     r.cg = op.cg;
     r.cvg = op.cvg;
     switch (op.kind) {
-      case K_DENSE1:
-        r.kind = (op.cl || op.cg) ? Q_U1C : Q_U1;
-        r.a = regidx(rs, op.t);
+      case K_DENSE1: {
+        const uint32_t b = regidx(rs, op.t);
         for (int e = 0; e < 4; ++e) {
           r.u.m[2 * e] = pool[op.off + e].real();
           r.u.m[2 * e + 1] = pool[op.off + e].imag();
         }
-        break;
-      case K_MCX:
-      case K_MCY:
-        r.kind = op.kind == K_MCX ? Q_X : Q_Y;
-        r.a = regidx(rs, op.t);
-        // X with at most one control, on a register bit: register moves only
-        if (op.kind == K_MCX && op.cg == 0 && (op.cl & ~rs) == 0 && __builtin_popcount(op.cl) <= 1) {
-          r.kind = Q_XR;
-          r.b = op.cl ? regidx(rs, static_cast<uint32_t>(__builtin_ctz(op.cl))) : 0xFFu;
-          r.extra = (op.cvl & op.cl) ? 1u : 0u;
-          r.cl = r.cvl = 0;
+        if (op.cl || op.cg) {
+          need_identity(rs, {b});
+          r.kind = Q_U1C;
+          r.a = b;
+        } else if (identity_on(b)) {
+          r.kind = Q_U1;
+          r.a = b;
+        } else {
+          // relabelled qubit: pairs {j, j ^ m}, roles from the map
+          const uint32_t m = phys_of(1u << b), f = row_of(b), cb = (cinv >> b) & 1u;
+          const uint32_t hb = 31u - static_cast<uint32_t>(__builtin_clz(m));
+          uint32_t roles = 0, k = 0;
+          for (uint32_t j = 0; j < (1u << R); ++j) {
+            if (j & (1u << hb)) continue;
+            if ((static_cast<uint32_t>(__builtin_popcount(j & f)) & 1u) ^ cb) roles |= 1u << k;
+            ++k;
+          }
+          r.kind = Q_U1M;
+          r.a = m;
+          r.extra = roles;
         }
         break;
-      case K_DENSE:
+      }
+      case K_MCX:
+      case K_MCY: {
+        const uint32_t t = regidx(rs, op.t);
+        if (op.kind == K_MCX && op.cg == 0 && (op.cl & ~rs) == 0 && __builtin_popcount(op.cl) <= 1) {
+          // X with at most one register control: relabel the registers
+          if (op.cl == 0) {
+            cinv ^= 1u << t;
+          } else {
+            const uint32_t c = regidx(rs, static_cast<uint32_t>(__builtin_ctz(op.cl)));
+            const bool v = (op.cvl & op.cl) != 0;
+            auto cxm = [&](uint32_t y) { return y ^ (((y >> c) & 1u) << t); };
+            for (uint32_t q = 0; q < R; ++q) colinv[q] = cxm(colinv[q]);
+            cinv = cxm(cinv) ^ (v ? 0u : (1u << t));
+          }
+          map_dirty = true;
+          return;
+        }
+        need_identity(rs, {t});
+        r.kind = op.kind == K_MCX ? Q_X : Q_Y;
+        r.a = t;
+        break;
+      }
+      case K_DENSE: {
+        const uint32_t b0 = regidx(rs, op.pos[0]), b1 = regidx(rs, op.pos[1]);
+        need_identity(rs, {b0, b1});
         r.kind = Q_U2;
-        r.a = regidx(rs, op.pos[0]);
-        r.b = regidx(rs, op.pos[1]);
+        r.a = b0;
+        r.b = b1;
         r.extra = op.off;
         break;
+      }
       case K_DIAG:
         if (op.k == 1 && op.pos[0] != kNotLocal) {
           r.kind = Q_D1;
@@ -927,10 +1041,10 @@ void register_program(const std::vector<TileOp> &in, uint32_t B, uint32_t R, uin
       case K_PROBE: {
         r.kind = Q_PROBE;
         r.extra = op.npairs;
-        uint32_t xr = 0;
+        uint32_t xl = 0;
         for (uint32_t b = 0; b < B; ++b)
-          if (op.xl & (1u << b)) xr |= 1u << regidx(rs, b);
-        r.u.pr.xr = xr;
+          if (op.xl & (1u << b)) xl |= 1u << regidx(rs, b);
+        r.u.pr.xr = phys_of(xl);
         r.u.pr.zl = op.zl;
         r.u.pr.zg = op.zg;
         r.u.pr.pr = op.pr;
@@ -940,6 +1054,7 @@ void register_program(const std::vector<TileOp> &in, uint32_t B, uint32_t R, uin
       default:
         throw std::logic_error("aerhip: op cannot run in registers");
     }
+    stamp(r);
     rops.push_back(r);
   };
   auto emit_lds = [&](const TileOp &op) {

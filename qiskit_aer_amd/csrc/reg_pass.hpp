@@ -19,6 +19,7 @@
 // applied to the tile in LDS (Q_SPILL, TileOp primitives, Q_FILL).
 #pragma once
 
+#include <cstddef>
 #include <cstring>
 #include <utility>
 
@@ -43,9 +44,11 @@ enum RegPassOpKind : uint32_t {
   Q_DIAG = 53,      // diagonal of TileOp extra (local and fixed bits)
   Q_PROBE = 54,     // Pauli probe whose X/Y bits are register bits (u.pr)
   Q_NOP = 55,       // keeps the amplitudes (pads a register phase to an even length)
-  Q_XR = 56,        // X on register bit a controlled by register bit b with value extra
-                    // (b = 0xFF: no control): register moves only
+  Q_U1M = 57,       // 2x2 matrix on the register pairs {j, j ^ a}; bit k of extra set:
+                    // the k-th pair's lower register holds the |1> amplitude
 };
+// Flag on kind: the op carries a new register relabelling (map_bm, map_t0).
+constexpr uint32_t F_MAP = 0x80000000u;
 
 // Bit positions packed four to a word (byte i of word w is entry 4w+i), so
 // that the device reads them with scalar loads.
@@ -79,7 +82,9 @@ struct alignas(128) ROp {
   uint32_t extra;    // probe slot, TileOp index or pool offset
   uint32_t cl, cvl;  // local control mask and value
   uint64_t cg, cvg;  // controls on qubits fixed by the tile
-  uint32_t pad[6];
+  uint16_t map_bm[5];  // with F_MAP in kind: the relabelling in force from this op on
+  uint16_t map_t0;
+  uint32_t pad[3];
   union {
     T m[8];          // 2x2 complex matrix (column major), diagonal, phase
     float mr[16];    // complex64 Q_U1: the matrix as a real 4x4, rows over
@@ -95,7 +100,7 @@ static_assert(sizeof(ROp<double>) == 128 && sizeof(ROp<float>) == 128, "ROp is o
 inline void rops_to_float(const ROp<double> *in, size_t n, ROp<float> *out) {
   for (size_t i = 0; i < n; ++i) {
     std::memcpy(&out[i], &in[i], sizeof(ROp<float>));
-    const uint32_t k = in[i].kind;
+    const uint32_t k = in[i].kind & ~F_MAP;
     if (k == Q_U1) {
       const double *m = in[i].u.m;  // m0 = u00, m1 = u10, m2 = u01, m3 = u11 (re, im)
       const double u00r = m[0], u00i = m[1], u10r = m[2], u10i = m[3];
@@ -105,7 +110,7 @@ inline void rops_to_float(const ROp<double> *in, size_t n, ROp<float> *out) {
                                u10r, -u10i, u11r, -u11i,   // re a1'
                                u10i, u10r, u11i, u11r};    // im a1'
       for (int e = 0; e < 16; ++e) out[i].u.mr[e] = static_cast<float>(rows[e]);
-    } else if (k == Q_U1C || k == Q_D1 || k == Q_PHASE) {
+    } else if (k == Q_U1C || k == Q_U1M || k == Q_D1 || k == Q_PHASE) {
       for (int e = 0; e < 8; ++e) out[i].u.m[e] = static_cast<float>(in[i].u.m[e]);
     }
   }
@@ -134,19 +139,23 @@ __device__ __forceinline__ cx<T> csel(bool c, cx<T> a, cx<T> b) {
   return mk<T>(c ? a.x : b.x, c ? a.y : b.y);
 }
 
-// Uniform layout state: the tile-bit mask of each register bit; the tile
-// index of register j is the OR of the masks of j's bits (at(j), folded at
-// compile time since j always is a constant).  R words instead of 2^R keep
-// the scalar register file free.
+// Uniform layout state: register j holds the amplitude whose tile index is
+// at(j) = t0 ^ XOR of bm[s] over the set bits s of j (folded at compile time
+// since j always is a constant).  In a fresh layout bm[s] is the tile bit of
+// register bit s and t0 = 0; X and CX gates between register qubits are
+// applied by changing bm/t0 (an affine relabelling of the registers) instead
+// of moving amplitudes.  R + 1 words instead of 2^R keep the scalar register
+// file free.
 template <int NR>
 struct RegIdx {
   static constexpr int R = NR == 8 ? 3 : NR == 16 ? 4 : 5;
-  uint32_t bm[R];
+  uint32_t bm[R];  // tile-bit mask contributed by register bit s
+  uint32_t t0;     // tile bits set for every register (a pending X)
   __device__ __forceinline__ uint32_t at(int j) const {
-    uint32_t v = 0;
+    uint32_t v = t0;
 #pragma unroll
     for (int r = 0; r < R; ++r)
-      if ((j >> r) & 1) v |= bm[r];
+      if ((j >> r) & 1) v ^= bm[r];
     return v;
   }
 };
@@ -161,7 +170,26 @@ __device__ __forceinline__ RegIdx<1 << R> reg_index(const RLayout &lay) {
   RegIdx<1 << R> r;
 #pragma unroll
   for (int b = 0; b < R; ++b) r.bm[b] = 1u << byte_at(lay.rp, b);
+  r.t0 = 0;
   return r;
+}
+
+// HBM offsets of the registers under a relabelling: the tile bits of at(j)
+// mapped to their qubits (only register tile bits occur in at(j)).
+template <int R>
+__device__ __forceinline__ RegOff<1 << R> reg_offset_mapped(const RLayout &lay,
+                                                            const RegIdx<1 << R> &ri) {
+  RegOff<1 << R> o;
+#pragma unroll
+  for (int j = 0; j < (1 << R); ++j) {
+    const uint32_t l = ri.at(j);
+    uint64_t g = 0;
+#pragma unroll
+    for (int b = 0; b < R; ++b)
+      if ((l >> byte_at(lay.rp, b)) & 1u) g |= 1ull << byte_at(lay.rq, b);
+    o.gr[j] = g;
+  }
+  return o;
 }
 
 template <int R>
@@ -212,6 +240,44 @@ __device__ __forceinline__ Amps<T, NR> q_u1(const Amps<T, NR> &a, cx<T> m0, cx<T
     o.v[j1] = mv2(m1, a.v[j], m3, a.v[j1]);
   }
   return o;
+}
+
+// 2x2 gate on the register pairs {j, j ^ m} (m: a relabelled qubit spans
+// several register bits).  The k-th pair (k-th register j with the top bit of
+// m clear) is |0>,|1> = (j, j^m) if bit k of roles is 0, else (j^m, j).
+template <int m>
+constexpr int top_bit() {
+  return m >= 16 ? 4 : m >= 8 ? 3 : m >= 4 ? 2 : m >= 2 ? 1 : 0;
+}
+
+template <typename T, int NR, int m>
+__device__ __forceinline__ Amps<T, NR> q_u1m(const Amps<T, NR> &a, uint32_t roles, cx<T> m0,
+                                             cx<T> m1, cx<T> m2, cx<T> m3) {
+  constexpr int hb = top_bit<m>();
+  Amps<T, NR> o;
+  int k = 0;
+#pragma unroll
+  for (int j = 0; j < NR; ++j) {
+    if (j & (1 << hb)) continue;
+    const int jj = j ^ m;
+    const bool sw = (roles >> k) & 1u;
+    const cx<T> c00 = sw ? m3 : m0, c01 = sw ? m1 : m2, c10 = sw ? m2 : m1, c11 = sw ? m0 : m3;
+    o.v[j] = mv2(c00, a.v[j], c01, a.v[jj]);
+    o.v[jj] = mv2(c10, a.v[j], c11, a.v[jj]);
+    ++k;
+  }
+  return o;
+}
+
+template <typename T, int NR, int M = 1>
+__device__ __forceinline__ Amps<T, NR> d_u1m(uint32_t m, uint32_t roles, const Amps<T, NR> &a,
+                                             cx<T> m0, cx<T> m1, cx<T> m2, cx<T> m3) {
+  if constexpr (M == NR - 1) {
+    return q_u1m<T, NR, M>(a, roles, m0, m1, m2, m3);
+  } else {
+    if (m == M) return q_u1m<T, NR, M>(a, roles, m0, m1, m2, m3);
+    return d_u1m<T, NR, M + 1>(m, roles, a, m0, m1, m2, m3);
+  }
 }
 
 // complex64 2x2 gate with MFMA: v_mfma_f32_4x4x1f32 runs 16 independent 4x4
@@ -289,16 +355,6 @@ __device__ __forceinline__ Amps<T, NR> q_xy(const Amps<T, NR> &a, uint32_t tlc,
     o.v[j] = csel(on, n0, a0);
     o.v[j1] = csel(on, n1, a1);
   }
-  return o;
-}
-
-// X on register bit t where register bit c equals cv (c < 0: always): moves only.
-template <typename T, int NR, int t, int c, int cv>
-__device__ __forceinline__ Amps<T, NR> q_xr(const Amps<T, NR> &a) {
-  Amps<T, NR> o;
-#pragma unroll
-  for (int j = 0; j < NR; ++j)
-    o.v[j] = (c < 0 || ((j >> (c < 0 ? 0 : c)) & 1) == cv) ? a.v[j ^ (1 << t)] : a.v[j];
   return o;
 }
 
@@ -384,31 +440,6 @@ __device__ __forceinline__ Amps<T, NR> d_xy(uint32_t b, const Amps<T, NR> &a, ui
   }
 }
 
-// X on register bit t: control register bit c (C enumerates; c == t or
-// 0xFF: no control), control value cv.
-template <typename T, int NR, int t, int C = 0>
-__device__ __forceinline__ Amps<T, NR> d_xr_c(uint32_t c, uint32_t cv, const Amps<T, NR> &a) {
-  if constexpr (C == reg_bits_of<NR>()) {
-    return q_xr<T, NR, t, -1, 0>(a);
-  } else if constexpr (C == t) {
-    return d_xr_c<T, NR, t, C + 1>(c, cv, a);
-  } else {
-    if (c == C) return cv ? q_xr<T, NR, t, C, 1>(a) : q_xr<T, NR, t, C, 0>(a);
-    return d_xr_c<T, NR, t, C + 1>(c, cv, a);
-  }
-}
-
-template <typename T, int NR, int B0 = 0>
-__device__ __forceinline__ Amps<T, NR> d_xr(uint32_t t, uint32_t c, uint32_t cv,
-                                            const Amps<T, NR> &a) {
-  if constexpr (B0 == reg_bits_of<NR>() - 1) {
-    return d_xr_c<T, NR, B0>(c, cv, a);
-  } else {
-    if (t == B0) return d_xr_c<T, NR, B0>(c, cv, a);
-    return d_xr<T, NR, B0 + 1>(t, c, cv, a);
-  }
-}
-
 // 4x4 on (b0, b1), enumerated as P = b0 * R + b1 over distinct pairs.
 template <typename T, int NR, int P = 1>
 __device__ __forceinline__ Amps<T, NR> d_u2(uint32_t p, const Amps<T, NR> &a, uint32_t tlc,
@@ -474,14 +505,46 @@ __device__ __forceinline__ LayoutState<1 << R> make_state(const RLayout &lay, ui
   return s;
 }
 
-// The first 16 bytes of an op record (kind, register bits, extra), read ahead.
+// The relabelling carried by an op with F_MAP, read as three dwords (the
+// 16-bit fields sit at a 4-byte aligned offset, so scalar loads can fetch them).
+template <int R, typename T>
+__device__ __forceinline__ void load_map(const ROp<T> &op, RegIdx<1 << R> &ri) {
+  static_assert(offsetof(ROp<T>, map_bm) % 4 == 0 && offsetof(ROp<T>, map_t0) == offsetof(ROp<T>, map_bm) + 10,
+                "map fields are three aligned dwords");
+  uint32_t w[3];
+  __builtin_memcpy(w, op.map_bm, sizeof(w));
+#pragma unroll
+  for (int s = 0; s < R; ++s) ri.bm[s] = (w[s >> 1] >> (16 * (s & 1))) & 0xFFFFu;
+  ri.t0 = w[2] >> 16;
+}
+
+// A wave-uniform 64-bit value moved to scalar registers.
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v >> 32));
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+// The value itself, hidden from the optimizer (no code is emitted).
+__device__ __forceinline__ uint32_t opaque_u32(uint32_t v) {
+  asm volatile("" : "+s"(v));
+  return v;
+}
+
+// The scalar head of an op and, for complex64, the lane's row of the real
+// 4x4 matrix that the MFMA path of Q_U1 multiplies by (row lane % 4; a
+// per-lane vector load, issued one op ahead like the head).
 struct OpHead {
   uint32_t kind, a, b, extra;
+  f32x4 arow;
 };
 
 template <typename T>
 __device__ __forceinline__ OpHead load_head(const ROp<T> &op) {
-  return OpHead{op.kind, op.a, op.b, op.extra};
+  OpHead h{op.kind, op.a, op.b, op.extra, f32x4{0.f, 0.f, 0.f, 0.f}};
+  if constexpr (sizeof(T) == 4)
+    h.arow = *reinterpret_cast<const f32x4 *>(op.u.mr + 4 * (threadIdx.x & 3u));
+  return h;
 }
 
 // A register op that rewrites every amplitude (a layout change or a gate):
@@ -505,7 +568,25 @@ __device__ __forceinline__ Amps<T, 1 << R> rw_op(const ROp<T> &op, const OpHead 
                                                  const ProbeSink &sink) {
   constexpr int NR = 1 << R;
   constexpr int TB = B - R;
-  const uint32_t kind = hd.kind, ra = hd.a, rb = hd.b, extra = hd.extra;
+  const uint32_t kind = hd.kind & ~F_MAP, ra = hd.a, rb = hd.b, extra = hd.extra;
+  if (hd.kind & F_MAP) {
+    load_map<R>(op, st.ri);
+  }
+  // Q_U1, the most frequent kind, is tested first, on a copy of the kind
+  // that the compiler cannot fold into the comparison tree of the others.
+  if (opaque_u32(kind) == Q_U1) {
+    if constexpr (sizeof(T) == 4) {
+      // A operand of issue k: row (lane % 4), column k of the real 4x4 matrix
+      return d_u1_mfma<NR>(ra, a, hd.arow.x, hd.arow.y, hd.arow.z, hd.arow.w);
+    } else {
+      const cx<T> m0 = mat_at<T>(op.u.m, 0), m1 = mat_at<T>(op.u.m, 1), m2 = mat_at<T>(op.u.m, 2), m3 = mat_at<T>(op.u.m, 3);
+      return d_u1<T, NR>(ra, a, m0, m1, m2, m3);
+    }
+  }
+  if (kind == Q_U1M) {
+    const cx<T> m0 = mat_at<T>(op.u.m, 0), m1 = mat_at<T>(op.u.m, 1), m2 = mat_at<T>(op.u.m, 2), m3 = mat_at<T>(op.u.m, 3);
+    return d_u1m<T, NR>(ra, extra, a, m0, m1, m2, m3);
+  }
   if (kind == Q_PROBE || kind == Q_NOP) {
     if (kind == Q_PROBE) {
       const RProbe &pr = op.u.pr;
@@ -531,23 +612,6 @@ __device__ __forceinline__ Amps<T, 1 << R> rw_op(const ROp<T> &op, const OpHead 
   const uint32_t cl = tile_ok ? op.cl : 0u;
   const uint32_t cvl = tile_ok ? op.cvl : 1u;
   const uint32_t tlc = st.tl & cl;
-  if (kind == Q_U1) {
-    if constexpr (sizeof(T) == 4) {
-      // A operand of issue k: row (lane % 4), column k of the real 4x4 matrix
-      const uint32_t r = threadIdx.x & 3u;
-      const bool r0 = r == 0, r1 = r == 1, r2 = r == 2;
-      const float *mr = op.u.mr;
-      const float A0 = r0 ? mr[0] : r1 ? mr[4] : r2 ? mr[8] : mr[12];
-      const float A1 = r0 ? mr[1] : r1 ? mr[5] : r2 ? mr[9] : mr[13];
-      const float A2 = r0 ? mr[2] : r1 ? mr[6] : r2 ? mr[10] : mr[14];
-      const float A3 = r0 ? mr[3] : r1 ? mr[7] : r2 ? mr[11] : mr[15];
-      return d_u1_mfma<NR>(ra, a, A0, A1, A2, A3);
-    } else {
-      const cx<T> m0 = mat_at<T>(op.u.m, 0), m1 = mat_at<T>(op.u.m, 1), m2 = mat_at<T>(op.u.m, 2), m3 = mat_at<T>(op.u.m, 3);
-      return d_u1<T, NR>(ra, a, m0, m1, m2, m3);
-    }
-  }
-  if (kind == Q_XR) return d_xr<T, NR>(ra, rb, extra, a);
   if (kind == Q_U1C) {
     const cx<T> m0 = mat_at<T>(op.u.m, 0), m1 = mat_at<T>(op.u.m, 1), m2 = mat_at<T>(op.u.m, 2), m3 = mat_at<T>(op.u.m, 3);
     return d_u1c<T, NR>(ra, a, tlc, st.ri, cl, cvl, m0, m1, m2, m3);
@@ -617,6 +681,7 @@ __device__ __forceinline__ void reg_tile(cx<T> *__restrict__ sv, const PassDesc 
     const uint32_t q = pd->spos[i];
     base = ((base >> q) << (q + 1)) | (base & ((1ull << q) - 1ull));
   }
+  base = uniform64(base);  // keeps the tile predicates of the ops on the scalar unit
   const ROp<T> *opl = rops + pd->op_begin;
   uint32_t o = 0;
   for (;;) {
@@ -638,18 +703,23 @@ __device__ __forceinline__ void reg_tile(cx<T> *__restrict__ sv, const PassDesc 
     // Register ops alternate between banks a and b.  The head of the op
     // after the current one is loaded while the current op runs.
     const uint32_t cnt = e.extra;
-    OpHead h0 = load_head(opl[o]);
+    const ROp<T> *opp = opl + o;
+    OpHead h0 = load_head(opp[0]);
     for (uint32_t i = 0; i < cnt; i += 2) {
-      const OpHead h1 = load_head(opl[o + 1]);
-      const Amps<T, NR> b = rw_op<T, B, R>(opl[o], h0, a, st, lds, red, base, tid, tops, pool, sink);
-      h0 = load_head(opl[o + 2]);
-      a = rw_op<T, B, R>(opl[o + 1], h1, b, st, lds, red, base, tid, tops, pool, sink);
-      o += 2;
+      const OpHead h1 = load_head(opp[1]);
+      const Amps<T, NR> b = rw_op<T, B, R>(opp[0], h0, a, st, lds, red, base, tid, tops, pool, sink);
+      h0 = load_head(opp[2]);
+      a = rw_op<T, B, R>(opp[1], h1, b, st, lds, red, base, tid, tops, pool, sink);
+      opp += 2;
     }
+    o += cnt;
     const ROp<T> &l = opl[o++];
-    if (l.kind == Q_STORE) {
+    if (l.kind & F_MAP) {
+      load_map<R>(l, st.ri);
+    }
+    if ((l.kind & ~F_MAP) == Q_STORE) {
       if (l.extra) return;  // read-only pass
-      const RegOff<NR> ro = reg_offset<R>(l.u.lay);
+      const RegOff<NR> ro = reg_offset_mapped<R>(l.u.lay, st.ri);
       cx<T> *p = sv + base + thread_offset<TB>(l.u.lay, tid);
 #pragma unroll
       for (int j = 0; j < NR; ++j) p[ro.gr[j]] = a.v[j];
@@ -670,9 +740,20 @@ __device__ __forceinline__ void reg_tile(cx<T> *__restrict__ sv, const PassDesc 
   }
 }
 
+// Waves per SIMD that the tile's LDS allows (160 KiB per CU, four SIMDs;
+// 1 KiB per workgroup for the small arrays beside the tile); register
+// allocation is held to that occupancy.
+template <typename T, int B, int R>
+constexpr int reg_pass_waves() {
+  const int blocks = (160 * 1024) / ((static_cast<int>(sizeof(cx<T>)) << B) + 1024);
+  const int waves = blocks * ((1 << (B - R)) / 64) / 4;
+  return waves < 1 ? 1 : waves > 8 ? 8 : waves;
+}
+
 // One workgroup per tile.
 template <typename T, int B, int R>
-__global__ __launch_bounds__(1 << (B - R)) void k_reg_pass(
+__global__ __launch_bounds__(1 << (B - R))
+__attribute__((amdgpu_waves_per_eu(reg_pass_waves<T, B, R>()))) void k_reg_pass(
     cx<T> *__restrict__ sv, const PassDesc *__restrict__ pd, const ROp<T> *__restrict__ rops,
     const TileOp *__restrict__ tops, const cx<double> *__restrict__ pool,
     double *__restrict__ probes) {
