@@ -50,7 +50,7 @@ __attribute__((amdgpu_waves_per_eu(reg_pass_waves<T, B, R>()))) void k_batch(cx<
     const uint64_t ntiles = 1ull << pd->ncomp;
     if (pd->kind == 2) {
       for (uint64_t t = 0; t < ntiles; ++t)
-        reg_tile<T, B, R, 16>(sv, pd, rops, tops, pool, sink, t, lds, mat, doff, red);
+        reg_tile<T, B, R, 16, true>(sv, pd, rops, tops, pool, sink, t, lds, mat, doff, red);
     } else if (pd->kind == 3) {
       const TileOp &op = tops[pd->op_begin];
       for (uint64_t t = 0; t < ntiles; ++t) {

@@ -1310,6 +1310,17 @@ void Engine::emit_program(Program &prog) {
   pool_.clear();
 }
 
+namespace {
+template <typename T, int B, int R>
+void launch_reg_pass(bool lean, uint32_t grid, hipStream_t stream, cx<T> *sv, const PassDesc *pd,
+                     const ROp<T> *rops, const TileOp *tops, const cx<double> *pool, double *probes) {
+  if (lean)
+    k_reg_pass<T, B, R, false><<<grid, 1 << (B - R), 0, stream>>>(sv, pd, rops, tops, pool, probes);
+  else
+    k_reg_pass<T, B, R, true><<<grid, 1 << (B - R), 0, stream>>>(sv, pd, rops, tops, pool, probes);
+}
+}  // namespace
+
 void Engine::flush() {
   if (queue_.empty()) return;
   if (plan_only_) throw std::logic_error("aerhip: a planning handle cannot execute");
@@ -1379,24 +1390,22 @@ void Engine::flush() {
     const PassDesc &pd = passes[i];
     if (pd.kind == 2) {
       const uint32_t grid = static_cast<uint32_t>(1ull << pd.ncomp);
+      bool lean = true;
+      for (uint32_t k = 0; k < pd.nops && lean; ++k) lean = lean_kind(rops[pd.op_begin + k].kind);
       if (prec_ == AERHIP_PRECISION_DOUBLE) {
         auto *sv = static_cast<cx<double> *>(sv_);
         auto *ro = static_cast<const ROp<double> *>(drops);
         if (reg_bits_ == 3)
-          k_reg_pass<double, kTileBitsC128, 3><<<grid, 1 << (kTileBitsC128 - 3), 0, stream_>>>(
-              sv, dpass + i, ro, dops, dpool, dprobes);
+          launch_reg_pass<double, kTileBitsC128, 3>(lean, grid, stream_, sv, dpass + i, ro, dops, dpool, dprobes);
         else
-          k_reg_pass<double, kTileBitsC128, 4><<<grid, 1 << (kTileBitsC128 - 4), 0, stream_>>>(
-              sv, dpass + i, ro, dops, dpool, dprobes);
+          launch_reg_pass<double, kTileBitsC128, 4>(lean, grid, stream_, sv, dpass + i, ro, dops, dpool, dprobes);
       } else {
         auto *sv = static_cast<cx<float> *>(sv_);
         auto *ro = static_cast<const ROp<float> *>(drops);
         if (reg_bits_ == 4)
-          k_reg_pass<float, kTileBitsC64, 4><<<grid, 1 << (kTileBitsC64 - 4), 0, stream_>>>(
-              sv, dpass + i, ro, dops, dpool, dprobes);
+          launch_reg_pass<float, kTileBitsC64, 4>(lean, grid, stream_, sv, dpass + i, ro, dops, dpool, dprobes);
         else
-          k_reg_pass<float, kTileBitsC64, 5><<<grid, 1 << (kTileBitsC64 - 5), 0, stream_>>>(
-              sv, dpass + i, ro, dops, dpool, dprobes);
+          launch_reg_pass<float, kTileBitsC64, 5>(lean, grid, stream_, sv, dpass + i, ro, dops, dpool, dprobes);
       }
     } else if (pd.kind == 3) {
       // probe sweep over the whole state, one workgroup per tile-sized slice

@@ -50,6 +50,13 @@ enum RegPassOpKind : uint32_t {
 // Flag on kind: the op carries a new register relabelling (map_bm, map_t0).
 constexpr uint32_t F_MAP = 0x80000000u;
 
+// Kinds the lean register-pass variant runs.
+constexpr bool lean_kind(uint32_t kind) {
+  const uint32_t k = kind & ~F_MAP;
+  return k == Q_LOAD || k == Q_STORE || k == Q_RELAYOUT || k == Q_U1 || k == Q_U1M || k == Q_X ||
+         k == Q_D1 || k == Q_PHASE || k == Q_NOP;
+}
+
 // Bit positions packed four to a word (byte i of word w is entry 4w+i), so
 // that the device reads them with scalar loads.
 struct RLayout {
@@ -558,7 +565,7 @@ __device__ __forceinline__ cx<T> mat_at(const T *m, int e) {
   return mk<T>(m[2 * e], m[2 * e + 1]);
 }
 
-template <typename T, int B, int R>
+template <typename T, int B, int R, bool FULL>
 __device__ __forceinline__ Amps<T, 1 << R> rw_op(const ROp<T> &op, const OpHead &hd,
                                                  const Amps<T, 1 << R> &a,
                                                  LayoutState<1 << R> &st, cx<T> *lds,
@@ -588,7 +595,7 @@ __device__ __forceinline__ Amps<T, 1 << R> rw_op(const ROp<T> &op, const OpHead 
     return d_u1m<T, NR>(ra, extra, a, m0, m1, m2, m3);
   }
   if (kind == Q_PROBE || kind == Q_NOP) {
-    if (kind == Q_PROBE) {
+    if (FULL && kind == Q_PROBE) {
       const RProbe &pr = op.u.pr;
       const uint32_t sgb = __popcll(base & pr.zg) & 1u;
       double acc = q_probe_dispatch<T, NR, 0>(pr.xr, a, st.tl, st.ri, pr.zl, sgb, pr.pr, pr.pi);
@@ -612,35 +619,63 @@ __device__ __forceinline__ Amps<T, 1 << R> rw_op(const ROp<T> &op, const OpHead 
   const uint32_t cl = tile_ok ? op.cl : 0u;
   const uint32_t cvl = tile_ok ? op.cvl : 1u;
   const uint32_t tlc = st.tl & cl;
-  if (kind == Q_U1C) {
+  if (FULL && kind == Q_U1C) {
     const cx<T> m0 = mat_at<T>(op.u.m, 0), m1 = mat_at<T>(op.u.m, 1), m2 = mat_at<T>(op.u.m, 2), m3 = mat_at<T>(op.u.m, 3);
     return d_u1c<T, NR>(ra, a, tlc, st.ri, cl, cvl, m0, m1, m2, m3);
   }
-  if (kind == Q_X || kind == Q_Y) {
-    const bool isy = kind == Q_Y;
+  if (kind == Q_X || (FULL && kind == Q_Y)) {
+    const bool isy = FULL && kind == Q_Y;
+    if (cl == 0) {
+      // No local control: the gate acts on the whole tile or not at all, so
+      // the registers are only moved (no per-lane selects).
+      if (cvl != 0) return a;
+      return d_xy<T, NR>(ra, a, 0u, st.ri, 0u, 0u, isy);
+    }
     return d_xy<T, NR>(ra, a, tlc, st.ri, cl, cvl, isy);
   }
-  if (kind == Q_U2) return d_u2<T, NR>(ra * reg_bits_of<NR>() + rb, a, tlc, st.ri, cl, cvl, pool + extra);
+  if (FULL && kind == Q_U2) return d_u2<T, NR>(ra * reg_bits_of<NR>() + rb, a, tlc, st.ri, cl, cvl, pool + extra);
   const cx<T> one = mk<T>(1, 0);
   Amps<T, NR> o;
-  if (kind == Q_D1) {
-    const cx<T> d0 = mat_at<T>(op.u.m, 0), d1 = mat_at<T>(op.u.m, 1);
-    const uint32_t t = ra;
+  // Diagonal kinds.  The predicate (l & cl) == cvl, l = tl | at(j), splits
+  // into a lane part (thread tile bits) and a register part (the register
+  // tile bits rbits), which is uniform per register j.
+  if (kind == Q_D1 || kind == Q_PHASE) {
+    if (cvl & ~cl) return a;  // a tile-level control is not met
+    uint32_t rbits = 0;
 #pragma unroll
-    for (int j = 0; j < NR; ++j) {
-      const uint32_t l = st.tl | st.ri.at(j);
-      const cx<T> d = ((l & cl) == cvl) ? csel(((l >> t) & 1u) != 0, d1, d0) : one;
-      o.v[j] = cmul(a.v[j], d);
+    for (int s = 0; s < R; ++s) rbits |= st.ri.bm[s];
+    const uint32_t clr = cl & rbits, clt = cl & ~rbits;
+    const bool lane_ok = (st.tl & clt) == (cvl & clt);
+    if (kind == Q_D1) {
+      const cx<T> d0 = mat_at<T>(op.u.m, 0), d1 = mat_at<T>(op.u.m, 1);
+      const uint32_t t = ra;
+      if (cl == 0) {
+        if ((rbits >> t) & 1u) {
+          // target on a register bit: the factor of each register is uniform
+#pragma unroll
+          for (int j = 0; j < NR; ++j) o.v[j] = cmul(a.v[j], ((st.ri.at(j) >> t) & 1u) ? d1 : d0);
+        } else {
+          const cx<T> dl = csel(((st.tl >> t) & 1u) != 0, d1, d0);
+#pragma unroll
+          for (int j = 0; j < NR; ++j) o.v[j] = cmul(a.v[j], dl);
+        }
+        return o;
+      }
+#pragma unroll
+      for (int j = 0; j < NR; ++j) {
+        const uint32_t l = st.tl | st.ri.at(j);
+        const cx<T> d = lane_ok && (st.ri.at(j) & clr) == (cvl & clr) ? csel(((l >> t) & 1u) != 0, d1, d0) : one;
+        o.v[j] = cmul(a.v[j], d);
+      }
+      return o;
     }
-    return o;
-  }
-  if (kind == Q_PHASE) {
+    // Q_PHASE
     const cx<T> ph = mat_at<T>(op.u.m, 0);
 #pragma unroll
-    for (int j = 0; j < NR; ++j)
-      o.v[j] = cmul(a.v[j], csel((tlc | (st.ri.at(j) & cl)) == cvl, ph, one));
+    for (int j = 0; j < NR; ++j) o.v[j] = cmul(a.v[j], csel((tlc | (st.ri.at(j) & cl)) == cvl, ph, one));
     return o;
   }
+  if constexpr (!FULL) return a;  // not reached: lean programs hold no other kinds
   // Q_DIAG (the last kind, so that every path computes new amplitudes)
   const TileOp &top = tops[extra];
   const uint32_t k = top.k;
@@ -663,7 +698,7 @@ __device__ __forceinline__ Amps<T, 1 << R> rw_op(const ROp<T> &op, const OpHead 
 // form
 //     LOAD  { register ops }  ( SPILL { LDS ops } FILL { register ops } )*  STORE
 // where each register phase holds an even number of ops (Q_NOP pads).
-template <typename T, int B, int R, int MATCAP>
+template <typename T, int B, int R, int MATCAP, bool FULL>
 __device__ __forceinline__ void reg_tile(cx<T> *__restrict__ sv, const PassDesc *__restrict__ pd,
                                          const ROp<T> *__restrict__ rops,
                                          const TileOp *__restrict__ tops,
@@ -707,9 +742,9 @@ __device__ __forceinline__ void reg_tile(cx<T> *__restrict__ sv, const PassDesc 
     OpHead h0 = load_head(opp[0]);
     for (uint32_t i = 0; i < cnt; i += 2) {
       const OpHead h1 = load_head(opp[1]);
-      const Amps<T, NR> b = rw_op<T, B, R>(opp[0], h0, a, st, lds, red, base, tid, tops, pool, sink);
+      const Amps<T, NR> b = rw_op<T, B, R, FULL>(opp[0], h0, a, st, lds, red, base, tid, tops, pool, sink);
       h0 = load_head(opp[2]);
-      a = rw_op<T, B, R>(opp[1], h1, b, st, lds, red, base, tid, tops, pool, sink);
+      a = rw_op<T, B, R, FULL>(opp[1], h1, b, st, lds, red, base, tid, tops, pool, sink);
       opp += 2;
     }
     o += cnt;
@@ -717,7 +752,7 @@ __device__ __forceinline__ void reg_tile(cx<T> *__restrict__ sv, const PassDesc 
     if (l.kind & F_MAP) {
       load_map<R>(l, st.ri);
     }
-    if ((l.kind & ~F_MAP) == Q_STORE) {
+    if (!FULL || (l.kind & ~F_MAP) == Q_STORE) {
       if (l.extra) return;  // read-only pass
       const RegOff<NR> ro = reg_offset_mapped<R>(l.u.lay, st.ri);
       cx<T> *p = sv + base + thread_offset<TB>(l.u.lay, tid);
@@ -750,8 +785,10 @@ constexpr int reg_pass_waves() {
   return waves < 1 ? 1 : waves > 8 ? 8 : waves;
 }
 
-// One workgroup per tile.
-template <typename T, int B, int R>
+// One workgroup per tile.  FULL = false: a lean variant for programs whose
+// ops are all LOAD, STORE, RELAYOUT, U1, U1M, X, D1, PHASE and NOP (a single
+// register phase); the smaller op loop keeps the frequent gates faster.
+template <typename T, int B, int R, bool FULL>
 __global__ __launch_bounds__(1 << (B - R))
 __attribute__((amdgpu_waves_per_eu(reg_pass_waves<T, B, R>()))) void k_reg_pass(
     cx<T> *__restrict__ sv, const PassDesc *__restrict__ pd, const ROp<T> *__restrict__ rops,
@@ -762,7 +799,7 @@ __attribute__((amdgpu_waves_per_eu(reg_pass_waves<T, B, R>()))) void k_reg_pass(
   __shared__ uint32_t doff[32];
   __shared__ double red[(1 << (B - R)) / 64];
   const ProbeSink sink{probes, blockIdx.x, gridDim.x, false};
-  reg_tile<T, B, R, 16>(sv, pd, rops, tops, pool, sink, blockIdx.x, lds, mat, doff, red);
+  reg_tile<T, B, R, 16, FULL>(sv, pd, rops, tops, pool, sink, blockIdx.x, lds, mat, doff, red);
 }
 
 }  // namespace aerhip
