@@ -25,12 +25,13 @@ ROOT = pathlib.Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
+FP64_PEAK_TFLOPS = 78.6  # MI355X published FP64 peak (vector and matrix)
 
 # HBM bytes per tile-pass launch of the default workload (30 qubits, c128),
 # from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (profiles/reg_pass_30q_brickwork_c128.md):
-# 2.0 x 8 388 816 KiB fetched (FETCH_SIZE counts half the bytes, calibrated on a
+# 2.0 x 8 388 833 KiB fetched (FETCH_SIZE counts half the bytes, calibrated on a
 # kernel that reads the 16 GiB state once) + 16 777 216 KiB written.
-MEASURED_TRAFFIC_30Q_C128 = (2 * 8388816 + 16777216) * 1024
+MEASURED_TRAFFIC_30Q_C128 = (2 * 8388833 + 16777216) * 1024
 
 
 def log(msg: str) -> None:
@@ -275,6 +276,10 @@ def main() -> int:
     circ = circuits.brickwork_u3_cx(n, args.layers, seed, measure=False)
     log(f"bench: brickwork {n}q depth {args.layers} c128, {circ.count_gates()} gates")
     c128 = run_engine(n, "double", circ, args.steps, args.warmup)
+    # FP64 work of the step: each U3 costs 8 fused multiply-adds per amplitude
+    # (a complex 2x2 on each pair), CX only moves amplitudes.
+    u3_per_step = n * args.layers
+    fp64_tflops = u3_per_step * (1 << n) * 16 / (c128["ms_per_step"] / 1e3) / 1e12
     log(f"bench: c128 {c128['gates_per_s']:.1f} gates/s, {c128['hbm_gbs']:.0f} GB/s, "
         f"kernel {c128['kernel_gbs']:.0f} GB/s, {c128['passes_per_step']:.0f} passes, "
         f"{c128['gathers_per_step']:.0f} register layouts per step")
@@ -344,6 +349,10 @@ def main() -> int:
                      "unit": "GB/s", "frac": c128["kernel_gbs"] / HBM_PEAK_GBS,
                      "traffic": MEASURED_TRAFFIC_30Q_C128 if (n, args.layers) == (30, 40) else None,
                      "algorithmic_bytes_per_launch": c128["sweep_bytes"]},
+        # the same step against the FP64 vector peak (the pass's other bound)
+        "fp64_compute": {"achieved": fp64_tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": fp64_tflops / FP64_PEAK_TFLOPS,
+                         "flops_per_step": u3_per_step * (1 << n) * 16},
         "cpu_baseline": cpu,
         "extra": extra,
     }
