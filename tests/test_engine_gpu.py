@@ -331,3 +331,50 @@ def test_pauli_strings_wider_than_a_tile(precision):
     o2 = [o.expval_pauli(qs, lab, 0.3 - 0.4j) for lab in labels]
     got = e.saved_expvals()
     assert np.max(np.abs(np.array(got) - np.array(o2))) < (1e-12 if precision == "double" else 1e-5)
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_register_relabelling_random_mix(precision, seed):
+    """Random mixes that drive the register relabelling: X/CX between
+    register qubits (absorbed), 2x2 gates on relabelled qubits, diagonals and
+    phases on relabelled and thread bits, CX controlled by a qubit outside the
+    tile (a whole-tile X), controlled 2x2 and 4x4 gates (which need a fresh
+    layout), and saved Pauli values read through a relabelled layout."""
+    n = 16
+    print(f"case: relabelling mix {precision} seed={seed}", flush=True)
+    e, o, rng = pair(n, 700 + seed, precision)
+    near = [0, 1, 2, 3, 4, 5]
+    far = [12, 15]
+    for _ in range(240):
+        kind = rng.integers(0, 10)
+        a, b = (int(x) for x in rng.choice(near, 2, replace=False))
+        if kind <= 2:
+            e.apply_mcx([a, b]); o.apply_mcx([a, b])
+        elif kind == 3:
+            e.apply_mcx([b]); o.apply_mcx([b])
+        elif kind <= 5:
+            m = D.random_unitary(2, rng)
+            e.apply_matrix([a], m); o.apply_matrix([a], m)
+        elif kind == 6:
+            t = rng.uniform(-3, 3)
+            e.apply_rotation([a], "z", t); o.apply_rotation([a], "z", t)
+            ph = np.exp(1j * rng.uniform(-3, 3))
+            e.apply_mcphase([a, b], ph); o.apply_mcphase([a, b], ph)
+        elif kind == 7:
+            c = int(rng.choice(far))
+            e.apply_mcx([c, a]); o.apply_mcx([c, a])
+        elif kind == 8:
+            u = D.random_unitary(2, rng)
+            e.apply_mcu([b, a], u); o.apply_mcu([b, a], u)
+        else:
+            m = D.random_unitary(4, rng)
+            e.apply_matrix([a, b], m); o.apply_matrix([a, b], m)
+    if seed == 2:
+        labels = ["XYZIXZ", "ZZIIYX", "IXXYZY"]
+        for lab in labels:
+            e.save_expval_pauli(near, lab)
+        got = e.saved_expvals()
+        want = [o.expval_pauli(near, lab) for lab in labels]
+        assert np.max(np.abs(np.array(got) - np.array(want))) <= 10 * TOL[precision]
+    close(e, o, precision)
