@@ -32,12 +32,13 @@ HIP_FLAGS = [
     "-fPIC",
     "-Wall",
     "-Wno-unused-result",
+    "-pthread",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-mllvm", "-amdgpu-mfma-vgpr-form",
     "-mllvm", "-simplifycfg-sink-common=false",
     f"-I{ROOT / 'include'}",
 ]
-CXX_FLAGS = ["-std=c++17", "-O2", "-fPIC", "-Wall", f"-I{ROOT / 'include'}"]
+CXX_FLAGS = ["-std=c++17", "-O2", "-fPIC", "-Wall", "-pthread", f"-I{ROOT / 'include'}"]
 
 
 def _run(cmd: list[str], verbose: bool) -> None:
@@ -70,7 +71,7 @@ def build(verbose: bool = True) -> pathlib.Path:
         _run(["g++", *CXX_FLAGS, "-c", str(src), "-o", str(obj)], verbose)
         objs.append(str(obj))
     tmp = OUT.with_suffix(".so.tmp")
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", str(tmp)], verbose)
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", *objs, "-o", str(tmp)], verbose)
     os.replace(tmp, OUT)
     return OUT
 

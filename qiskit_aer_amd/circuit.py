@@ -8,6 +8,7 @@ them into the ``aerhip_op`` wire format of include/aerhip.h.
 from __future__ import annotations
 
 import ctypes as C
+import itertools
 from dataclasses import dataclass, field
 from typing import Optional, Sequence
 
@@ -115,36 +116,83 @@ class Circuit:
 
     # wire format ---------------------------------------------------------------
     def pack(self):
-        """Returns (ctypes array of AerhipOp, keepalive list)."""
-        arr = (L.AerhipOp * len(self.ops))()
+        """Returns (ctypes array of AerhipOp, keepalive list).
+
+        The qubits, real parameters and names of all ops go into three flat
+        buffers and the records are filled column by column, so packing costs
+        a few Python-level passes over the op list rather than a handful of
+        small arrays per op."""
+        ops = self.ops
+        nops = len(ops)
+        arr = (L.AerhipOp * nops)()
         keep = []
-        for i, op in enumerate(self.ops):
-            q = np.ascontiguousarray(np.asarray(op.qubits, dtype=np.uint64))
-            keep.append(q)
-            a = arr[i]
-            a.type = op.type
-            a.num_qubits = q.size
-            name = op.name.encode()
-            keep.append(name)
-            a.name = name
-            a.qubits = q.ctypes.data_as(C.POINTER(C.c_uint64))
-            if op.type == L.OP_GATE:
-                p = np.ascontiguousarray(np.asarray(op.params, dtype=np.float64))
-                a.num_params = p.size
-            elif op.matrix is not None:
-                p = np.ascontiguousarray(op.matrix.astype(np.complex128)).view(np.float64)
-                a.num_params = op.matrix.size
-            else:
-                p = np.zeros(0, dtype=np.float64)
-                a.num_params = 0
-            keep.append(p)
-            a.params = p.ctypes.data_as(C.POINTER(C.c_double))
+        if nops == 0:
+            return arr, keep
+        rec = np.frombuffer(arr, dtype=_OP_DTYPE)
+        types = np.fromiter((o.type for o in ops), dtype=np.int32, count=nops)
+        rec["type"] = types
+        # qubits
+        nq = np.fromiter((len(o.qubits) for o in ops), dtype=np.int64, count=nops)
+        qflat = np.fromiter(itertools.chain.from_iterable(o.qubits for o in ops), dtype=np.uint64,
+                            count=int(nq.sum()))
+        qflat = np.concatenate([qflat, np.zeros(1, dtype=np.uint64)])
+        keep.append(qflat)
+        qoff = np.concatenate([[0], np.cumsum(nq)[:-1]])
+        rec["num_qubits"] = nq
+        rec["qubits"] = np.uint64(qflat.ctypes.data) + (qoff * 8).astype(np.uint64)
+        # real parameters: gate angles, or complex matrices as (re, im) pairs
+        is_gate = types == L.OP_GATE
+        mats = [i for i in range(nops) if not is_gate[i] and ops[i].matrix is not None]
+        npar = np.fromiter((len(o.params) if o.type == L.OP_GATE else 0 for o in ops), dtype=np.int64,
+                           count=nops)
+        for i in mats:
+            npar[i] = 2 * ops[i].matrix.size
+        poff = np.concatenate([[0], np.cumsum(npar)[:-1]])
+        pflat = np.zeros(int(npar.sum()) + 1, dtype=np.float64)
+        gate_idx = np.nonzero(is_gate & (npar > 0))[0]
+        if gate_idx.size:
+            vals = np.fromiter(itertools.chain.from_iterable(ops[i].params for i in gate_idx),
+                               dtype=np.float64, count=int(npar[gate_idx].sum()))
+            starts = poff[gate_idx]
+            lens = npar[gate_idx]
+            pos = np.repeat(starts - np.concatenate([[0], np.cumsum(lens)[:-1]]), lens) + np.arange(vals.size)
+            pflat[pos] = vals
+        for i in mats:
+            pflat[poff[i]:poff[i] + npar[i]] = np.ascontiguousarray(
+                ops[i].matrix.astype(np.complex128)).view(np.float64)
+        keep.append(pflat)
+        rec["num_params"] = np.where(is_gate, npar, npar // 2)
+        rec["params"] = np.uint64(pflat.ctypes.data) + (poff * 8).astype(np.uint64)
+        # names: one NUL-terminated copy of each distinct name
+        names = sorted({o.name for o in ops})
+        blob = b"".join(nm.encode() + b"\0" for nm in names)
+        nbuf = np.frombuffer(blob + b"\0", dtype=np.uint8)
+        keep.append(nbuf)
+        addr, at = {}, 0
+        for nm in names:
+            addr[nm] = nbuf.ctypes.data + at
+            at += len(nm.encode()) + 1
+        rec["name"] = np.fromiter((addr[o.name] for o in ops), dtype=np.uint64, count=nops)
+        # Pauli labels and measure clbits (rare)
+        for i, op in enumerate(ops):
             if op.label is not None:
                 lab = op.label.encode()
                 keep.append(lab)
-                a.string_param = lab
+                arr[i].string_param = lab
             if op.memory:
                 m = np.ascontiguousarray(np.asarray(op.memory, dtype=np.uint64))
                 keep.append(m)
-                a.memory = m.ctypes.data_as(C.POINTER(C.c_uint64))
+                arr[i].memory = m.ctypes.data_as(C.POINTER(C.c_uint64))
         return arr, keep
+
+
+# Field layout of AerhipOp (include/aerhip.h) for filling packed records.
+_OP_DTYPE = np.dtype({
+    "names": ["type", "num_qubits", "name", "qubits", "num_params", "reserved", "params",
+              "string_param", "memory"],
+    "formats": ["<i4", "<u4", "<u8", "<u8", "<u4", "<u4", "<u8", "<u8", "<u8"],
+    "offsets": [L.AerhipOp.type.offset, L.AerhipOp.num_qubits.offset, L.AerhipOp.name.offset,
+                L.AerhipOp.qubits.offset, L.AerhipOp.num_params.offset, L.AerhipOp.reserved.offset,
+                L.AerhipOp.params.offset, L.AerhipOp.string_param.offset, L.AerhipOp.memory.offset],
+    "itemsize": C.sizeof(L.AerhipOp),
+})

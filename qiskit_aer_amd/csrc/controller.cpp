@@ -24,6 +24,7 @@
 
 #include "aerhip.h"
 #include "gate_matrices.hpp"
+#include "host_parallel.hpp"
 
 // Sets the thread-local message returned by aerhip_last_error (engine.hip).
 extern "C" __attribute__((visibility("hidden"))) void aerhip_detail_set_error(const char *msg);
@@ -585,8 +586,9 @@ int aerhip_batch_pauli_expval(const aerhip_op *ops, const uint64_t *op_offsets,
         aerhip_qv *h = aerhip_detail_qv_create_planner(num_qubits[i], precision, device);
         if (!h) throw std::runtime_error(aerhip_last_error());
         handles.push_back(h);
-        queue_circuit(h, i);
       }
+      // Planning handles are independent: queue the circuits on host threads.
+      aerhip::parallel_for(num_circuits, [&](uint64_t i) { queue_circuit(handles[i], i); });
       std::vector<double> vals(term_offsets[num_circuits] - term_offsets[0] + 1);
       ck(aerhip_detail_batch_run(handles.data(), num_circuits, device, vals.data(), kernel_ms));
       for (uint64_t i = 0; i < num_circuits; ++i)

@@ -8,6 +8,7 @@
 #include "aerhip.h"
 #include "engine.hpp"
 #include "gate_matrices.hpp"
+#include "host_parallel.hpp"
 #include "kernels.hpp"
 #include "batch.hpp"
 #include "reg_pass.hpp"
@@ -1791,30 +1792,47 @@ void run_batch(Engine *const *engines, uint64_t count, int device, double *value
   if (!count) return;
   if (!engines || !values) throw std::invalid_argument("aerhip: null argument");
   const int prec = engines[0]->precision();
-  Program prog;
+  for (uint64_t i = 0; i < count; ++i)
+    if (!engines[i]->plan_only() || engines[i]->precision() != prec)
+      throw std::invalid_argument("aerhip: batch needs planning handles of one precision");
+  // Plan every circuit on the host threads (the handles are independent),
+  // then concatenate the programs in circuit order.
+  std::vector<Program> parts(count);
+  std::vector<uint32_t> nprobes(count);
+  parallel_for(count, [&](uint64_t i) {
+    nprobes[i] = static_cast<uint32_t>(engines[i]->probes_queued());
+    engines[i]->emit_program(parts[i]);
+  });
+  // Offsets of each circuit's passes / ops / register ops / pool entries in
+  // the concatenated program.
+  std::vector<uint64_t> pass_at(count + 1, 0), op_at(count + 1, 0), rop_at(count + 1, 0),
+      pool_at(count + 1, 0);
   std::vector<BatchCircuit> circ(count);
   std::vector<double> cost(count);
   uint64_t sv_off = 0, probe_off = 0;
   for (uint64_t i = 0; i < count; ++i) {
-    Engine &e = *engines[i];
-    if (!e.plan_only() || e.precision() != prec)
-      throw std::invalid_argument("aerhip: batch needs planning handles of one precision");
+    const Program &p = parts[i];
+    for (const PassDesc &d : p.passes)
+      if (d.kind == 1)
+        throw std::invalid_argument("aerhip: batched circuits take dense matrices on at most 5 qubits");
+    pass_at[i + 1] = pass_at[i] + p.passes.size();
+    op_at[i + 1] = op_at[i] + p.ops.size();
+    rop_at[i + 1] = rop_at[i] + p.rops.size();
+    pool_at[i + 1] = pool_at[i] + p.pool.size();
     BatchCircuit &c = circ[i];
     std::memset(&c, 0, sizeof(c));
     c.sv_off = sv_off;
-    c.num_qubits = e.num_qubits();
-    c.pass_begin = static_cast<uint32_t>(prog.passes.size());
+    c.num_qubits = engines[i]->num_qubits();
+    c.pass_begin = static_cast<uint32_t>(pass_at[i]);
+    c.pass_count = static_cast<uint32_t>(p.passes.size());
     c.probe_base = static_cast<uint32_t>(probe_off);
-    c.num_probes = static_cast<uint32_t>(e.probes_queued());
-    e.emit_program(prog);
-    c.pass_count = static_cast<uint32_t>(prog.passes.size()) - c.pass_begin;
-    for (uint32_t p = c.pass_begin; p < prog.passes.size(); ++p)
-      if (prog.passes[p].kind == 1)
-        throw std::invalid_argument("aerhip: batched circuits take dense matrices on at most 5 qubits");
+    c.num_probes = nprobes[i];
     cost[i] = static_cast<double>(c.pass_count + 1) * static_cast<double>(1ull << c.num_qubits);
     sv_off += align_up(1ull << c.num_qubits, 256);
     probe_off += c.num_probes;
   }
+  if (op_at[count] > UINT32_MAX || rop_at[count] > UINT32_MAX || pool_at[count] > UINT32_MAX)
+    throw std::invalid_argument("aerhip: batch program too large");
   // Largest circuits first: workgroups start in index order.
   std::vector<uint64_t> order(count);
   for (uint64_t i = 0; i < count; ++i) order[i] = i;
@@ -1825,20 +1843,49 @@ void run_batch(Engine *const *engines, uint64_t count, int device, double *value
   const size_t amp = prec == AERHIP_PRECISION_DOUBLE ? 16 : 8;
   const size_t circ_bytes = count * sizeof(BatchCircuit);
   const size_t pass_off = align_up(circ_bytes, 256);
-  const size_t ops_off = align_up(pass_off + prog.passes.size() * sizeof(PassDesc), 256);
-  const size_t rops_off = align_up(ops_off + prog.ops.size() * sizeof(TileOp), 256);
-  const size_t pool_off = align_up(rops_off + prog.rops.size() * sizeof(ROp<double>), 256);
-  const size_t total = pool_off + prog.pool.size() * sizeof(cd) + 16;
-  std::vector<char> h(total, 0);
-  std::memcpy(h.data(), sorted.data(), circ_bytes);
-  std::memcpy(h.data() + pass_off, prog.passes.data(), prog.passes.size() * sizeof(PassDesc));
-  std::memcpy(h.data() + ops_off, prog.ops.data(), prog.ops.size() * sizeof(TileOp));
-  if (prec == AERHIP_PRECISION_DOUBLE) {
-    std::memcpy(h.data() + rops_off, prog.rops.data(), prog.rops.size() * sizeof(ROp<double>));
-  } else {
-    rops_to_float(prog.rops.data(), prog.rops.size(), reinterpret_cast<ROp<float> *>(h.data() + rops_off));
-  }
-  if (!prog.pool.empty()) std::memcpy(h.data() + pool_off, prog.pool.data(), prog.pool.size() * sizeof(cd));
+  const size_t ops_off = align_up(pass_off + pass_at[count] * sizeof(PassDesc), 256);
+  const size_t rops_off = align_up(ops_off + op_at[count] * sizeof(TileOp), 256);
+  const size_t pool_off = align_up(rops_off + rop_at[count] * sizeof(ROp<double>), 256);
+  const size_t total = pool_off + pool_at[count] * sizeof(cd) + 16;
+  // The upload image, written in place: each circuit's part is copied to its
+  // offsets with its indices rebased (pass op ranges, TileOp pool offsets,
+  // the pool offsets and TileOp indices carried by register ops).
+  std::unique_ptr<char[]> h(new char[total]);
+  std::memcpy(h.get(), sorted.data(), circ_bytes);
+  auto *hp = reinterpret_cast<PassDesc *>(h.get() + pass_off);
+  auto *ho = reinterpret_cast<TileOp *>(h.get() + ops_off);
+  auto *hpool = reinterpret_cast<cd *>(h.get() + pool_off);
+  parallel_for(count, [&](uint64_t i) {
+    Program &p = parts[i];
+    const uint32_t ob = static_cast<uint32_t>(op_at[i]), rb = static_cast<uint32_t>(rop_at[i]),
+                   pb = static_cast<uint32_t>(pool_at[i]);
+    for (size_t k = 0; k < p.passes.size(); ++k) {
+      PassDesc d = p.passes[k];
+      d.op_begin += d.kind == 2 ? rb : ob;
+      hp[pass_at[i] + k] = d;
+    }
+    for (size_t k = 0; k < p.ops.size(); ++k) {
+      TileOp t = p.ops[k];
+      t.off += pb;
+      ho[op_at[i] + k] = t;
+    }
+    for (ROp<double> &r : p.rops) {
+      const uint32_t kk = r.kind & ~F_MAP;
+      if (kk == Q_U2) r.extra += pb;
+      else if (kk == Q_DIAG || kk == Q_LDS) r.extra += ob;
+    }
+    if (prec == AERHIP_PRECISION_DOUBLE)
+      std::memcpy(h.get() + rops_off + rop_at[i] * sizeof(ROp<double>), p.rops.data(),
+                  p.rops.size() * sizeof(ROp<double>));
+    else
+      rops_to_float(p.rops.data(), p.rops.size(),
+                    reinterpret_cast<ROp<float> *>(h.get() + rops_off) + rop_at[i]);
+    if (!p.pool.empty()) std::memcpy(hpool + pool_at[i], p.pool.data(), p.pool.size() * sizeof(cd));
+    std::vector<PassDesc>().swap(p.passes);  // free the part early
+    std::vector<TileOp>().swap(p.ops);
+    std::vector<ROp<double>>().swap(p.rops);
+    std::vector<cd>().swap(p.pool);
+  });
 
   AERHIP_CHECK(hipSetDevice(device));
   void *d_states = nullptr, *d_prog = nullptr;
@@ -1861,7 +1908,7 @@ void run_batch(Engine *const *engines, uint64_t count, int device, double *value
     AERHIP_CHECK(hipMalloc(&d_states, sv_off * amp));
     AERHIP_CHECK(hipMalloc(&d_prog, total));
     AERHIP_CHECK(hipMalloc(reinterpret_cast<void **>(&d_probes), (probe_off + 1) * sizeof(double)));
-    AERHIP_CHECK(hipMemcpyAsync(d_prog, h.data(), total, hipMemcpyHostToDevice, st));
+    AERHIP_CHECK(hipMemcpyAsync(d_prog, h.get(), total, hipMemcpyHostToDevice, st));
     char *d = static_cast<char *>(d_prog);
     const auto *dc = reinterpret_cast<const BatchCircuit *>(d);
     const auto *dp = reinterpret_cast<const PassDesc *>(d + pass_off);
