@@ -525,6 +525,18 @@ __device__ __forceinline__ void load_map(const ROp<T> &op, RegIdx<1 << R> &ri) {
   ri.t0 = w[2] >> 16;
 }
 
+// The amplitudes unchanged, as the result of an op that leaves them alone.
+// The empty asm pins the copy into the other register bank to this path:
+// without it the compiler hoists the copy into the shared dispatch blocks,
+// where every op would pay for it.
+template <typename T, int NR>
+__device__ __forceinline__ Amps<T, NR> pass_through(const Amps<T, NR> &a) {
+  Amps<T, NR> o = a;
+#pragma unroll
+  for (int j = 0; j < NR; ++j) asm volatile("" : "+v"(o.v[j].x), "+v"(o.v[j].y));
+  return o;
+}
+
 // A wave-uniform 64-bit value moved to scalar registers.
 __device__ __forceinline__ uint64_t uniform64(uint64_t v) {
   const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v));
@@ -602,7 +614,7 @@ __device__ __forceinline__ Amps<T, 1 << R> rw_op(const ROp<T> &op, const OpHead 
       acc = block_sum_nt<(1 << TB)>(acc, red);
       if (tid == 0) sink.put(extra, acc);
     }
-    return a;
+    return pass_through(a);
   }
   if (kind == Q_RELAYOUT) {
     __syncthreads();
@@ -628,7 +640,7 @@ __device__ __forceinline__ Amps<T, 1 << R> rw_op(const ROp<T> &op, const OpHead 
     if (cl == 0) {
       // No local control: the gate acts on the whole tile or not at all, so
       // the registers are only moved (no per-lane selects).
-      if (cvl != 0) return a;
+      if (cvl != 0) return pass_through(a);
       return d_xy<T, NR>(ra, a, 0u, st.ri, 0u, 0u, isy);
     }
     return d_xy<T, NR>(ra, a, tlc, st.ri, cl, cvl, isy);
@@ -640,7 +652,7 @@ __device__ __forceinline__ Amps<T, 1 << R> rw_op(const ROp<T> &op, const OpHead 
   // into a lane part (thread tile bits) and a register part (the register
   // tile bits rbits), which is uniform per register j.
   if (kind == Q_D1 || kind == Q_PHASE) {
-    if (cvl & ~cl) return a;  // a tile-level control is not met
+    if (cvl & ~cl) return pass_through(a);  // a tile-level control is not met
     uint32_t rbits = 0;
 #pragma unroll
     for (int s = 0; s < R; ++s) rbits |= st.ri.bm[s];
@@ -675,7 +687,7 @@ __device__ __forceinline__ Amps<T, 1 << R> rw_op(const ROp<T> &op, const OpHead 
     for (int j = 0; j < NR; ++j) o.v[j] = cmul(a.v[j], csel((tlc | (st.ri.at(j) & cl)) == cvl, ph, one));
     return o;
   }
-  if constexpr (!FULL) return a;  // not reached: lean programs hold no other kinds
+  if constexpr (!FULL) return pass_through(a);  // not reached: lean programs hold no other kinds
   // Q_DIAG (the last kind, so that every path computes new amplitudes)
   const TileOp &top = tops[extra];
   const uint32_t k = top.k;
