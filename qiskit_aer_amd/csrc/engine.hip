@@ -760,7 +760,8 @@ void validate_register_program(const ROp<double> *p, size_t n, uint32_t B, uint3
     for (uint32_t i = 0; i < e.extra; ++i) {
       const ROp<double> &r = p[o++];
       check_map(r);
-      switch (r.kind & ~F_MAP) {
+      if ((r.kind & F_TX) && (r.kind & ~F_ALL) != Q_U1M) fail("folded X flag on a kind other than Q_U1M");
+      switch (r.kind & ~F_ALL) {
         case Q_RELAYOUT:
           check_layout(r.u.lay);
           regmask = 0;
@@ -769,7 +770,7 @@ void validate_register_program(const ROp<double> *p, size_t n, uint32_t B, uint3
         case Q_U1M:
           if (r.a == 0 || r.a >= (1u << R) || r.extra >= (1u << ((1u << R) / 2))) fail("relabelled 2x2 gate");
           break;
-        case Q_U1: case Q_U1C: case Q_X: case Q_Y:
+        case Q_U1: case Q_U1X: case Q_U1C: case Q_X: case Q_Y:
           if (r.a >= R) fail("register bit of a gate");
           break;
         case Q_U2:
@@ -916,7 +917,7 @@ void register_program(const std::vector<TileOp> &in, uint32_t B, uint32_t R, uin
     r.map_t0 = static_cast<uint16_t>(tile_mask(cinv));
     map_dirty = false;
   };
-  auto emit = [&](uint32_t kind, uint32_t rs) {
+  auto emit_raw = [&](uint32_t kind, uint32_t rs) {
     if (kind == Q_STORE || kind == Q_SPILL) {
       uint32_t cnt = static_cast<uint32_t>(rops.size() - enter_at - 1);
       if (cnt & 1u) {
@@ -943,6 +944,46 @@ void register_program(const std::vector<TileOp> &in, uint32_t B, uint32_t R, uin
   auto regidx = [](uint32_t rs, uint32_t b) {
     return static_cast<uint32_t>(__builtin_popcount(rs & ((1u << b) - 1u)));
   };
+  // Logical register bits of the register tile bits in `tmask`.
+  auto reg_bits = [&](uint32_t tmask) {
+    uint32_t r = 0;
+    for (uint32_t b = 0; b < B; ++b)
+      if ((tmask & cur_rs & (1u << b))) r |= 1u << regidx(cur_rs, b);
+    return r;
+  };
+  // X gates controlled only by qubits fixed by the tile, waiting on logical
+  // register bits (pend) to fold into the next 2x2 gate on that bit (F_TX);
+  // anything else that touches the bit, and every layout change, first
+  // emits them as Q_X.
+  uint32_t pend = 0;
+  uint64_t pcg[8] = {}, pcvg[8] = {};
+  auto materialize = [&](uint32_t bits) {
+    bits &= pend;
+    if (!bits) return;
+    bool fresh = true;
+    for (uint32_t b = 0; b < R; ++b)
+      if ((bits >> b) & 1u) fresh = fresh && identity_on(b);
+    if (!fresh) {
+      emit_raw(Q_RELAYOUT, cur_rs);  // Q_X pairs the layout's own register bits
+      if (relayouts) ++*relayouts;
+    }
+    for (uint32_t b = 0; b < R; ++b) {
+      if (!((bits >> b) & 1u)) continue;
+      ROp<double> x;
+      std::memset(&x, 0, sizeof(x));
+      x.kind = Q_X;
+      x.a = b;
+      x.cg = pcg[b];
+      x.cvg = pcvg[b];
+      stamp(x);
+      rops.push_back(x);
+    }
+    pend &= ~bits;
+  };
+  auto emit = [&](uint32_t kind, uint32_t rs) {
+    if (kind == Q_RELAYOUT || kind == Q_STORE || kind == Q_SPILL) materialize(pend);
+    emit_raw(kind, rs);
+  };
   // Ops that need the registers in the layout's own order (pair structure on
   // their register bits) start from a fresh layout when relabelled.
   auto need_identity = [&](uint32_t rs, std::initializer_list<uint32_t> bits) {
@@ -966,12 +1007,22 @@ void register_program(const std::vector<TileOp> &in, uint32_t B, uint32_t R, uin
           r.u.m[2 * e] = pool[op.off + e].real();
           r.u.m[2 * e + 1] = pool[op.off + e].imag();
         }
+        materialize(reg_bits(op.cl));
+        bool fold = false;
+        if (!op.cl && !op.cg && ((pend >> b) & 1u)) {
+          // fold the waiting X: U X where its tile predicate holds
+          fold = true;
+          r.cg = pcg[b];
+          r.cvg = pcvg[b];
+          pend &= ~(1u << b);
+        }
         if (op.cl || op.cg) {
+          materialize(1u << b);
           need_identity(rs, {b});
           r.kind = Q_U1C;
           r.a = b;
         } else if (identity_on(b)) {
-          r.kind = Q_U1;
+          r.kind = fold ? Q_U1X : Q_U1;
           r.a = b;
         } else {
           // relabelled qubit: pairs {j, j ^ m}, roles from the map
@@ -983,7 +1034,7 @@ void register_program(const std::vector<TileOp> &in, uint32_t B, uint32_t R, uin
             if ((static_cast<uint32_t>(__builtin_popcount(j & f)) & 1u) ^ cb) roles |= 1u << k;
             ++k;
           }
-          r.kind = Q_U1M;
+          r.kind = fold ? (Q_U1M | F_TX) : Q_U1M;
           r.a = m;
           r.extra = roles;
         }
@@ -992,6 +1043,15 @@ void register_program(const std::vector<TileOp> &in, uint32_t B, uint32_t R, uin
       case K_MCX:
       case K_MCY: {
         const uint32_t t = regidx(rs, op.t);
+        if (op.kind == K_MCX && op.cl == 0 && op.cg != 0) {
+          // controls all fixed by the tile: wait for the next 2x2 on t
+          materialize(1u << t);
+          pend |= 1u << t;
+          pcg[t] = op.cg;
+          pcvg[t] = op.cvg;
+          return;
+        }
+        materialize((1u << t) | reg_bits(op.cl));
         if (op.kind == K_MCX && op.cg == 0 && (op.cl & ~rs) == 0 && __builtin_popcount(op.cl) <= 1) {
           // X with at most one register control: relabel the registers
           if (op.cl == 0) {
@@ -1013,6 +1073,7 @@ void register_program(const std::vector<TileOp> &in, uint32_t B, uint32_t R, uin
       }
       case K_DENSE: {
         const uint32_t b0 = regidx(rs, op.pos[0]), b1 = regidx(rs, op.pos[1]);
+        materialize((1u << b0) | (1u << b1) | reg_bits(op.cl));
         need_identity(rs, {b0, b1});
         r.kind = Q_U2;
         r.a = b0;
@@ -1021,6 +1082,7 @@ void register_program(const std::vector<TileOp> &in, uint32_t B, uint32_t R, uin
         break;
       }
       case K_DIAG:
+        materialize(pend);
         if (op.k == 1 && op.pos[0] != kNotLocal) {
           r.kind = Q_D1;
           r.a = op.pos[0];
@@ -1035,11 +1097,13 @@ void register_program(const std::vector<TileOp> &in, uint32_t B, uint32_t R, uin
         }
         break;
       case K_MCPHASE:
+        materialize(reg_bits(op.cl));
         r.kind = Q_PHASE;
         r.u.m[0] = op.pr;
         r.u.m[1] = op.pi;
         break;
       case K_PROBE: {
+        materialize(pend);
         r.kind = Q_PROBE;
         r.extra = op.npairs;
         uint32_t xl = 0;
@@ -1870,7 +1934,7 @@ void run_batch(Engine *const *engines, uint64_t count, int device, double *value
       ho[op_at[i] + k] = t;
     }
     for (ROp<double> &r : p.rops) {
-      const uint32_t kk = r.kind & ~F_MAP;
+      const uint32_t kk = r.kind & ~F_ALL;
       if (kk == Q_U2) r.extra += pb;
       else if (kk == Q_DIAG || kk == Q_LDS) r.extra += ob;
     }

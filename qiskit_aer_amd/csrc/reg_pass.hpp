@@ -46,15 +46,22 @@ enum RegPassOpKind : uint32_t {
   Q_NOP = 55,       // keeps the amplitudes (pads a register phase to an even length)
   Q_U1M = 57,       // 2x2 matrix on the register pairs {j, j ^ a}; bit k of extra set:
                     // the k-th pair's lower register holds the |1> amplitude
+  Q_U1X = 58,       // Q_U1 with an X on the same bit folded in where (base & cg) == cvg
 };
 // Flag on kind: the op carries a new register relabelling (map_bm, map_t0).
 constexpr uint32_t F_MAP = 0x80000000u;
+// Flag on Q_U1M (Q_U1X is the Q_U1 form): where the tile predicate
+// (base & cg) == cvg holds, an X on the gate's qubit comes first (the matrix
+// is applied as U X: its two columns swapped).  An X controlled by qubits
+// fixed by the tile folds into the next 2x2 gate on its target this way.
+constexpr uint32_t F_TX = 0x40000000u;
+constexpr uint32_t F_ALL = F_MAP | F_TX;
 
 // Kinds the lean register-pass variant runs.
 constexpr bool lean_kind(uint32_t kind) {
-  const uint32_t k = kind & ~F_MAP;
-  return k == Q_LOAD || k == Q_STORE || k == Q_RELAYOUT || k == Q_U1 || k == Q_U1M || k == Q_X ||
-         k == Q_D1 || k == Q_PHASE || k == Q_NOP;
+  const uint32_t k = kind & ~F_ALL;
+  return k == Q_LOAD || k == Q_STORE || k == Q_RELAYOUT || k == Q_U1 || k == Q_U1X || k == Q_U1M ||
+         k == Q_X || k == Q_D1 || k == Q_PHASE || k == Q_NOP;
 }
 
 // Bit positions packed four to a word (byte i of word w is entry 4w+i), so
@@ -107,8 +114,8 @@ static_assert(sizeof(ROp<double>) == 128 && sizeof(ROp<float>) == 128, "ROp is o
 inline void rops_to_float(const ROp<double> *in, size_t n, ROp<float> *out) {
   for (size_t i = 0; i < n; ++i) {
     std::memcpy(&out[i], &in[i], sizeof(ROp<float>));
-    const uint32_t k = in[i].kind & ~F_MAP;
-    if (k == Q_U1) {
+    const uint32_t k = in[i].kind & ~F_ALL;
+    if (k == Q_U1 || k == Q_U1X) {
       const double *m = in[i].u.m;  // m0 = u00, m1 = u10, m2 = u01, m3 = u11 (re, im)
       const double u00r = m[0], u00i = m[1], u10r = m[2], u10i = m[3];
       const double u01r = m[4], u01i = m[5], u11r = m[6], u11i = m[7];
@@ -577,6 +584,19 @@ __device__ __forceinline__ cx<T> mat_at(const T *m, int e) {
   return mk<T>(m[2 * e], m[2 * e + 1]);
 }
 
+// The 2x2 matrix of a Q_U1 / Q_U1M (column major), its columns swapped when
+// an X is folded in on this tile (fx).
+template <typename T>
+__device__ __forceinline__ void u1_matrix(const ROp<T> &op, bool fx, cx<T> &m0, cx<T> &m1, cx<T> &m2,
+                                          cx<T> &m3) {
+  const cx<T> c0 = mat_at<T>(op.u.m, 0), c1 = mat_at<T>(op.u.m, 1), c2 = mat_at<T>(op.u.m, 2),
+              c3 = mat_at<T>(op.u.m, 3);
+  m0 = fx ? c2 : c0;
+  m1 = fx ? c3 : c1;
+  m2 = fx ? c0 : c2;
+  m3 = fx ? c1 : c3;
+}
+
 template <typename T, int B, int R, bool FULL>
 __device__ __forceinline__ Amps<T, 1 << R> rw_op(const ROp<T> &op, const OpHead &hd,
                                                  const Amps<T, 1 << R> &a,
@@ -587,7 +607,7 @@ __device__ __forceinline__ Amps<T, 1 << R> rw_op(const ROp<T> &op, const OpHead 
                                                  const ProbeSink &sink) {
   constexpr int NR = 1 << R;
   constexpr int TB = B - R;
-  const uint32_t kind = hd.kind & ~F_MAP, ra = hd.a, rb = hd.b, extra = hd.extra;
+  const uint32_t kind = hd.kind & ~F_ALL, ra = hd.a, rb = hd.b, extra = hd.extra;
   if (hd.kind & F_MAP) {
     load_map<R>(op, st.ri);
   }
@@ -602,8 +622,23 @@ __device__ __forceinline__ Amps<T, 1 << R> rw_op(const ROp<T> &op, const OpHead 
       return d_u1<T, NR>(ra, a, m0, m1, m2, m3);
     }
   }
+  if (kind == Q_U1X) {
+    const bool fx = (base & op.cg) == op.cvg;
+    if constexpr (sizeof(T) == 4) {
+      // A operand of issue k: row (lane % 4), column k of the real 4x4
+      // matrix; U X swaps the columns that act on a0 and a1 (k and k ^ 2).
+      const float A0 = fx ? hd.arow.z : hd.arow.x, A1 = fx ? hd.arow.w : hd.arow.y;
+      const float A2 = fx ? hd.arow.x : hd.arow.z, A3 = fx ? hd.arow.y : hd.arow.w;
+      return d_u1_mfma<NR>(ra, a, A0, A1, A2, A3);
+    } else {
+      cx<T> m0, m1, m2, m3;
+      u1_matrix(op, fx, m0, m1, m2, m3);
+      return d_u1<T, NR>(ra, a, m0, m1, m2, m3);
+    }
+  }
   if (kind == Q_U1M) {
-    const cx<T> m0 = mat_at<T>(op.u.m, 0), m1 = mat_at<T>(op.u.m, 1), m2 = mat_at<T>(op.u.m, 2), m3 = mat_at<T>(op.u.m, 3);
+    cx<T> m0, m1, m2, m3;
+    u1_matrix(op, (hd.kind & F_TX) && (base & op.cg) == op.cvg, m0, m1, m2, m3);
     return d_u1m<T, NR>(ra, extra, a, m0, m1, m2, m3);
   }
   if (kind == Q_PROBE || kind == Q_NOP) {
