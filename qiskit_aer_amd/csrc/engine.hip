@@ -760,14 +760,13 @@ void validate_register_program(const ROp<double> *p, size_t n, uint32_t B, uint3
     for (uint32_t i = 0; i < e.extra; ++i) {
       const ROp<double> &r = p[o++];
       check_map(r);
-      if ((r.kind & F_TX) && (r.kind & ~F_ALL) != Q_U1M) fail("folded X flag on a kind other than Q_U1M");
       switch (r.kind & ~F_ALL) {
         case Q_RELAYOUT:
           check_layout(r.u.lay);
           regmask = 0;
           for (uint32_t q = 0; q < R; ++q) regmask |= 1u << byte_at(r.u.lay.rp, static_cast<int>(q));
           break;
-        case Q_U1M:
+        case Q_U1M: case Q_U1MX:
           if (r.a == 0 || r.a >= (1u << R) || r.extra >= (1u << ((1u << R) / 2))) fail("relabelled 2x2 gate");
           break;
         case Q_U1: case Q_U1X: case Q_U1C: case Q_X: case Q_Y:
@@ -952,7 +951,8 @@ void register_program(const std::vector<TileOp> &in, uint32_t B, uint32_t R, uin
     return r;
   };
   // X gates controlled only by qubits fixed by the tile, waiting on logical
-  // register bits (pend) to fold into the next 2x2 gate on that bit (F_TX);
+  // register bits (pend) to fold into the next 2x2 gate on that bit (Q_U1X,
+  // Q_U1MX);
   // anything else that touches the bit, and every layout change, first
   // emits them as Q_X.
   uint32_t pend = 0;
@@ -1034,7 +1034,7 @@ void register_program(const std::vector<TileOp> &in, uint32_t B, uint32_t R, uin
             if ((static_cast<uint32_t>(__builtin_popcount(j & f)) & 1u) ^ cb) roles |= 1u << k;
             ++k;
           }
-          r.kind = fold ? (Q_U1M | F_TX) : Q_U1M;
+          r.kind = fold ? Q_U1MX : Q_U1M;
           r.a = m;
           r.extra = roles;
         }

@@ -47,21 +47,21 @@ enum RegPassOpKind : uint32_t {
   Q_U1M = 57,       // 2x2 matrix on the register pairs {j, j ^ a}; bit k of extra set:
                     // the k-th pair's lower register holds the |1> amplitude
   Q_U1X = 58,       // Q_U1 with an X on the same bit folded in where (base & cg) == cvg
+  Q_U1MX = 59,      // Q_U1M with an X on the same logical bit folded in likewise
 };
 // Flag on kind: the op carries a new register relabelling (map_bm, map_t0).
 constexpr uint32_t F_MAP = 0x80000000u;
-// Flag on Q_U1M (Q_U1X is the Q_U1 form): where the tile predicate
-// (base & cg) == cvg holds, an X on the gate's qubit comes first (the matrix
-// is applied as U X: its two columns swapped).  An X controlled by qubits
-// fixed by the tile folds into the next 2x2 gate on its target this way.
-constexpr uint32_t F_TX = 0x40000000u;
-constexpr uint32_t F_ALL = F_MAP | F_TX;
+// Q_U1X / Q_U1MX: where the tile predicate (base & cg) == cvg holds, an X on
+// the gate's qubit comes first (the matrix is applied as U X: its two columns
+// swapped).  An X controlled by qubits fixed by the tile folds into the next
+// 2x2 gate on its target this way.
+constexpr uint32_t F_ALL = F_MAP;
 
 // Kinds the lean register-pass variant runs.
 constexpr bool lean_kind(uint32_t kind) {
   const uint32_t k = kind & ~F_ALL;
   return k == Q_LOAD || k == Q_STORE || k == Q_RELAYOUT || k == Q_U1 || k == Q_U1X || k == Q_U1M ||
-         k == Q_X || k == Q_D1 || k == Q_PHASE || k == Q_NOP;
+         k == Q_U1MX || k == Q_X || k == Q_D1 || k == Q_PHASE || k == Q_NOP;
 }
 
 // Bit positions packed four to a word (byte i of word w is entry 4w+i), so
@@ -101,7 +101,7 @@ struct alignas(128) ROp {
   uint32_t pad[3];
   union {
     T m[8];          // 2x2 complex matrix (column major), diagonal, phase
-    float mr[16];    // complex64 Q_U1: the matrix as a real 4x4, rows over
+    float mr[16];    // complex64 Q_U1 / Q_U1X / Q_U1M: the matrix as a real 4x4, rows over
                      // (re a0', im a0', re a1', im a1'), columns over (re a0, im a0, re a1, im a1)
     RLayout lay;
     RProbe pr;
@@ -109,13 +109,14 @@ struct alignas(128) ROp {
 };
 static_assert(sizeof(ROp<double>) == 128 && sizeof(ROp<float>) == 128, "ROp is one 128-byte record");
 
-// complex64 copy of a register program: matrices rounded to float; Q_U1
-// carries its matrix as the real 4x4 that the MFMA path multiplies by.
+// complex64 copy of a register program: matrices rounded to float; the 2x2
+// gates (Q_U1, Q_U1X, Q_U1M) carry their matrix as the real 4x4 that the MFMA
+// path multiplies by.
 inline void rops_to_float(const ROp<double> *in, size_t n, ROp<float> *out) {
   for (size_t i = 0; i < n; ++i) {
     std::memcpy(&out[i], &in[i], sizeof(ROp<float>));
     const uint32_t k = in[i].kind & ~F_ALL;
-    if (k == Q_U1 || k == Q_U1X) {
+    if (k == Q_U1 || k == Q_U1X || k == Q_U1M || k == Q_U1MX) {
       const double *m = in[i].u.m;  // m0 = u00, m1 = u10, m2 = u01, m3 = u11 (re, im)
       const double u00r = m[0], u00i = m[1], u10r = m[2], u10i = m[3];
       const double u01r = m[4], u01i = m[5], u11r = m[6], u11i = m[7];
@@ -124,7 +125,7 @@ inline void rops_to_float(const ROp<double> *in, size_t n, ROp<float> *out) {
                                u10r, -u10i, u11r, -u11i,   // re a1'
                                u10i, u10r, u11i, u11r};    // im a1'
       for (int e = 0; e < 16; ++e) out[i].u.mr[e] = static_cast<float>(rows[e]);
-    } else if (k == Q_U1C || k == Q_U1M || k == Q_D1 || k == Q_PHASE) {
+    } else if (k == Q_U1C || k == Q_D1 || k == Q_PHASE) {
       for (int e = 0; e < 8; ++e) out[i].u.m[e] = static_cast<float>(in[i].u.m[e]);
     }
   }
@@ -335,6 +336,67 @@ __device__ __forceinline__ Amps<float, NR> q_u1_mfma(const Amps<float, NR> &a, f
     o.v[j1] = mk<float>(acc[p][2], acc[p][3]);
   }
   return o;
+}
+
+// complex64 2x2 gate on the relabelled pairs {j, j ^ m} (Q_U1M) with MFMA.
+// A pair whose lower register holds |1> (role bit set) takes the matrix
+// P R P (P swaps the pair's two amplitudes): its A operand is row (lane ^ 2)
+// of R with the column halves swapped, `As`; the others take `An`.
+template <int NR, int m>
+__device__ __forceinline__ Amps<float, NR> q_u1m_mfma(const Amps<float, NR> &a, uint32_t roles,
+                                                      f32x4 An, f32x4 As) {
+  constexpr int hb = top_bit<m>();
+  constexpr int NP = NR / 2;
+  constexpr int G = NP < 4 ? NP : 4;  // pairs whose accumulation chains interleave
+  // lower register of pair p: the p-th index with bit hb clear
+  auto lo = [](int p) { return ((p >> hb) << (hb + 1)) | (p & ((1 << hb) - 1)); };
+  Amps<float, NR> o;
+#pragma unroll
+  for (int g = 0; g < NP; g += G) {
+    f32x4 acc[G];
+    f32x4 A[G];
+#pragma unroll
+    for (int p = 0; p < G; ++p) A[p] = ((roles >> (g + p)) & 1u) ? As : An;
+#pragma unroll
+    for (int p = 0; p < G; ++p)
+      acc[p] = __builtin_amdgcn_mfma_f32_4x4x1f32(A[p].x, a.v[lo(g + p)].x, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+#pragma unroll
+    for (int p = 0; p < G; ++p)
+      acc[p] = __builtin_amdgcn_mfma_f32_4x4x1f32(A[p].y, a.v[lo(g + p)].y, acc[p], 0, 0, 0);
+#pragma unroll
+    for (int p = 0; p < G; ++p)
+      acc[p] = __builtin_amdgcn_mfma_f32_4x4x1f32(A[p].z, a.v[lo(g + p) ^ m].x, acc[p], 0, 0, 0);
+#pragma unroll
+    for (int p = 0; p < G; ++p)
+      acc[p] = __builtin_amdgcn_mfma_f32_4x4x1f32(A[p].w, a.v[lo(g + p) ^ m].y, acc[p], 0, 0, 0);
+#pragma unroll
+    for (int p = 0; p < G; ++p) {
+      o.v[lo(g + p)] = mk<float>(acc[p][0], acc[p][1]);
+      o.v[lo(g + p) ^ m] = mk<float>(acc[p][2], acc[p][3]);
+    }
+  }
+  return o;
+}
+
+template <int NR, int M = 1>
+__device__ __forceinline__ Amps<float, NR> d_u1m_mfma(uint32_t m, uint32_t roles, const Amps<float, NR> &a,
+                                                      f32x4 An, f32x4 As) {
+  if constexpr (M == NR - 1) {
+    return q_u1m_mfma<NR, M>(a, roles, An, As);
+  } else {
+    if (m == M) return q_u1m_mfma<NR, M>(a, roles, An, As);
+    return d_u1m_mfma<NR, M + 1>(m, roles, a, An, As);
+  }
+}
+
+// Row (lane ^ 2) of a quad's A operands, read across the quad (DPP
+// quad_perm [2,3,0,1]), with its column halves swapped: the A operand of
+// P R P for the row this lane computes.
+__device__ __forceinline__ f32x4 mfma_row_swapped(f32x4 An) {
+  auto perm = [](float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false));
+  };
+  return f32x4{perm(An.z), perm(An.w), perm(An.x), perm(An.y)};
 }
 
 template <typename T, int NR, int b>
@@ -557,19 +619,35 @@ __device__ __forceinline__ uint32_t opaque_u32(uint32_t v) {
   return v;
 }
 
-// The scalar head of an op and, for complex64, the lane's row of the real
-// 4x4 matrix that the MFMA path of Q_U1 multiplies by (row lane % 4; a
-// per-lane vector load, issued one op ahead like the head).
+// The head of an op, read one op ahead: its scalar fields; for complex128
+// in the lean variant (PF), the 2x2 matrix of the payload as well (scalar
+// loads, so the matrix is in SGPRs when the op starts; the full variant has
+// no SGPRs to spare for it); for complex64, the lane's row of the real 4x4
+// matrix that the MFMA paths multiply by (row lane % 4, a per-lane vector
+// load).
+template <typename T, bool PF>
 struct OpHead {
   uint32_t kind, a, b, extra;
   f32x4 arow;
+  double m[8];
 };
 
-template <typename T>
-__device__ __forceinline__ OpHead load_head(const ROp<T> &op) {
-  OpHead h{op.kind, op.a, op.b, op.extra, f32x4{0.f, 0.f, 0.f, 0.f}};
-  if constexpr (sizeof(T) == 4)
+template <bool PF, typename T>
+__device__ __forceinline__ OpHead<T, PF> load_head(const ROp<T> &op) {
+  OpHead<T, PF> h;
+  h.kind = op.kind;
+  h.a = op.a;
+  h.b = op.b;
+  h.extra = op.extra;
+  h.arow = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int e = 0; e < 8; ++e) h.m[e] = 0.0;
+  if constexpr (sizeof(T) == 4) {
     h.arow = *reinterpret_cast<const f32x4 *>(op.u.mr + 4 * (threadIdx.x & 3u));
+  } else if constexpr (PF) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) h.m[e] = op.u.m[e];
+  }
   return h;
 }
 
@@ -584,13 +662,27 @@ __device__ __forceinline__ cx<T> mat_at(const T *m, int e) {
   return mk<T>(m[2 * e], m[2 * e + 1]);
 }
 
-// The 2x2 matrix of a Q_U1 / Q_U1M (column major), its columns swapped when
-// an X is folded in on this tile (fx).
-template <typename T>
-__device__ __forceinline__ void u1_matrix(const ROp<T> &op, bool fx, cx<T> &m0, cx<T> &m1, cx<T> &m2,
-                                          cx<T> &m3) {
-  const cx<T> c0 = mat_at<T>(op.u.m, 0), c1 = mat_at<T>(op.u.m, 1), c2 = mat_at<T>(op.u.m, 2),
-              c3 = mat_at<T>(op.u.m, 3);
+// The 2x2 matrix of a 2x2 gate op (column major: u00, u10, u01, u11), its
+// columns swapped when an X is folded in on this tile (fx).  complex64 ops
+// carry the real 4x4 form, whose rows 0 and 2 hold the real parts and rows 1
+// and 3 the imaginary parts.
+template <typename T, bool PF>
+__device__ __forceinline__ void u1_matrix(const ROp<T> &op, const OpHead<T, PF> &hd, bool fx, cx<T> &m0,
+                                          cx<T> &m1, cx<T> &m2, cx<T> &m3) {
+  cx<T> c0, c1, c2, c3;
+  if constexpr (sizeof(T) == 4) {
+    const float *r = op.u.mr;
+    c0 = mk<T>(r[0], r[4]);
+    c1 = mk<T>(r[8], r[12]);
+    c2 = mk<T>(r[2], r[6]);
+    c3 = mk<T>(r[10], r[14]);
+  } else {
+    const double *m = PF ? hd.m : reinterpret_cast<const double *>(op.u.m);
+    c0 = mat_at<double>(m, 0);
+    c1 = mat_at<double>(m, 1);
+    c2 = mat_at<double>(m, 2);
+    c3 = mat_at<double>(m, 3);
+  }
   m0 = fx ? c2 : c0;
   m1 = fx ? c3 : c1;
   m2 = fx ? c0 : c2;
@@ -598,7 +690,7 @@ __device__ __forceinline__ void u1_matrix(const ROp<T> &op, bool fx, cx<T> &m0, 
 }
 
 template <typename T, int B, int R, bool FULL>
-__device__ __forceinline__ Amps<T, 1 << R> rw_op(const ROp<T> &op, const OpHead &hd,
+__device__ __forceinline__ Amps<T, 1 << R> rw_op(const ROp<T> &op, const OpHead<T, !FULL> &hd,
                                                  const Amps<T, 1 << R> &a,
                                                  LayoutState<1 << R> &st, cx<T> *lds,
                                                  double *red, uint64_t base, uint32_t tid,
@@ -618,7 +710,8 @@ __device__ __forceinline__ Amps<T, 1 << R> rw_op(const ROp<T> &op, const OpHead 
       // A operand of issue k: row (lane % 4), column k of the real 4x4 matrix
       return d_u1_mfma<NR>(ra, a, hd.arow.x, hd.arow.y, hd.arow.z, hd.arow.w);
     } else {
-      const cx<T> m0 = mat_at<T>(op.u.m, 0), m1 = mat_at<T>(op.u.m, 1), m2 = mat_at<T>(op.u.m, 2), m3 = mat_at<T>(op.u.m, 3);
+      cx<T> m0, m1, m2, m3;
+      u1_matrix(op, hd, false, m0, m1, m2, m3);
       return d_u1<T, NR>(ra, a, m0, m1, m2, m3);
     }
   }
@@ -632,14 +725,29 @@ __device__ __forceinline__ Amps<T, 1 << R> rw_op(const ROp<T> &op, const OpHead 
       return d_u1_mfma<NR>(ra, a, A0, A1, A2, A3);
     } else {
       cx<T> m0, m1, m2, m3;
-      u1_matrix(op, fx, m0, m1, m2, m3);
+      u1_matrix(op, hd, fx, m0, m1, m2, m3);
       return d_u1<T, NR>(ra, a, m0, m1, m2, m3);
     }
   }
-  if (kind == Q_U1M) {
-    cx<T> m0, m1, m2, m3;
-    u1_matrix(op, (hd.kind & F_TX) && (base & op.cg) == op.cvg, m0, m1, m2, m3);
-    return d_u1m<T, NR>(ra, extra, a, m0, m1, m2, m3);
+  if (kind == Q_U1M) {  // relabelled pairs
+    if constexpr (sizeof(T) == 4 && !FULL) {
+      return d_u1m_mfma<NR>(ra, extra, a, hd.arow, mfma_row_swapped(hd.arow));
+    } else {
+      cx<T> m0, m1, m2, m3;
+      u1_matrix(op, hd, false, m0, m1, m2, m3);
+      return d_u1m<T, NR>(ra, extra, a, m0, m1, m2, m3);
+    }
+  }
+  if (kind == Q_U1MX) {  // the same, with an X folded in where its tile predicate holds
+    const bool fx = (base & op.cg) == op.cvg;
+    if constexpr (sizeof(T) == 4 && !FULL) {
+      const f32x4 An = fx ? f32x4{hd.arow.z, hd.arow.w, hd.arow.x, hd.arow.y} : hd.arow;
+      return d_u1m_mfma<NR>(ra, extra, a, An, mfma_row_swapped(An));
+    } else {
+      cx<T> m0, m1, m2, m3;
+      u1_matrix(op, hd, fx, m0, m1, m2, m3);
+      return d_u1m<T, NR>(ra, extra, a, m0, m1, m2, m3);
+    }
   }
   if (kind == Q_PROBE || kind == Q_NOP) {
     if (FULL && kind == Q_PROBE) {
@@ -786,11 +894,11 @@ __device__ __forceinline__ void reg_tile(cx<T> *__restrict__ sv, const PassDesc 
     // after the current one is loaded while the current op runs.
     const uint32_t cnt = e.extra;
     const ROp<T> *opp = opl + o;
-    OpHead h0 = load_head(opp[0]);
+    OpHead<T, !FULL> h0 = load_head<!FULL>(opp[0]);
     for (uint32_t i = 0; i < cnt; i += 2) {
-      const OpHead h1 = load_head(opp[1]);
+      const OpHead<T, !FULL> h1 = load_head<!FULL>(opp[1]);
       const Amps<T, NR> b = rw_op<T, B, R, FULL>(opp[0], h0, a, st, lds, red, base, tid, tops, pool, sink);
-      h0 = load_head(opp[2]);
+      h0 = load_head<!FULL>(opp[2]);
       a = rw_op<T, B, R, FULL>(opp[1], h1, b, st, lds, red, base, tid, tops, pool, sink);
       opp += 2;
     }
