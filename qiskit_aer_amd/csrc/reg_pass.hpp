@@ -953,8 +953,9 @@ __attribute__((amdgpu_waves_per_eu(reg_pass_waves<T, B, R>()))) void k_reg_pass(
   __shared__ cx<T> mat[16];
   __shared__ uint32_t doff[32];
   __shared__ double red[(1 << (B - R)) / 64];
-  const ProbeSink sink{probes, blockIdx.x, gridDim.x, false};
-  reg_tile<T, B, R, 16, FULL>(sv, pd, rops, tops, pool, sink, blockIdx.x, lds, mat, doff, red);
+  const uint64_t t = xcd_tile(blockIdx.x, gridDim.x);
+  const ProbeSink sink{probes, t, gridDim.x, false};
+  reg_tile<T, B, R, 16, FULL>(sv, pd, rops, tops, pool, sink, t, lds, mat, doff, red);
 }
 
 }  // namespace aerhip

@@ -14,6 +14,17 @@
 
 namespace aerhip {
 
+// Tile of this workgroup.  Workgroups are dealt to the 8 XCDs round-robin
+// (workgroup w runs on XCD w % 8); giving each XCD a contiguous range of
+// tiles keeps the concurrent accesses of one XCD (and its L2) on neighbouring
+// memory: +6-16 % sweep bandwidth on this part.  A bijection when the grid is
+// a multiple of 8 (the identity otherwise).
+__device__ __forceinline__ uint64_t xcd_tile(uint32_t w, uint32_t grid) {
+  if (grid & 7u) return w;
+  return static_cast<uint64_t>(w & 7u) * (grid >> 3) + (w >> 3);
+}
+
+
 // Register-resident amplitudes, passed by value so that every access uses a
 // compile-time index and the array stays in VGPRs.
 template <typename T, int NR>
@@ -348,8 +359,9 @@ __global__ __launch_bounds__(kThreads) void k_probe_sweep(const cx<T> *__restric
                                                           const TileOp *__restrict__ op,
                                                           double *__restrict__ probes) {
   __shared__ double red[kThreads / 64];
-  const double v = probe_sweep_block<T, kThreads>(sv, size, *op, blockIdx.x, gridDim.x, red);
-  if (threadIdx.x == 0) probes[static_cast<uint64_t>(op->npairs) * gridDim.x + blockIdx.x] = v;
+  const uint64_t t = xcd_tile(blockIdx.x, gridDim.x);
+  const double v = probe_sweep_block<T, kThreads>(sv, size, *op, t, gridDim.x, red);
+  if (threadIdx.x == 0) probes[static_cast<uint64_t>(op->npairs) * gridDim.x + t] = v;
 }
 
 // One workgroup per tile.
@@ -364,8 +376,9 @@ __global__ __launch_bounds__(kThreads) void k_tile_lds(cx<T> *__restrict__ sv,
   __shared__ cx<T> mat[1024];
   __shared__ uint32_t doff[32];
   __shared__ double red[kThreads / 64];
-  const ProbeSink sink{probes, blockIdx.x, gridDim.x, false};
-  lds_tile<T, kThreads, BMAX, 1024>(sv, pd, ops, pool, sink, blockIdx.x, tile, hioff, mat, doff,
+  const uint64_t t = xcd_tile(blockIdx.x, gridDim.x);
+  const ProbeSink sink{probes, t, gridDim.x, false};
+  lds_tile<T, kThreads, BMAX, 1024>(sv, pd, ops, pool, sink, t, tile, hioff, mat, doff,
                                     red);
 }
 
