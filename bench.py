@@ -299,6 +299,17 @@ def main() -> int:
         extra["qft_c128"] = {k: qres[k] for k in ("gates_per_s", "ms_per_step", "hbm_gbs",
                                                    "passes_per_step")}
         extra["qft_c128"]["gates"] = q.count_gates()
+        # Single-gate sweeps: one U3 per pass on a low and on the top qubit,
+        # the HBM-bound case (one read and one write of every amplitude).
+        for tq in (0, n - 1):
+            sw = circuits.Circuit(n)
+            sw.gate("u3", [tq], [0.3, 0.2, 0.1])
+            sres = run_engine(n, "double", sw, 20, 2)
+            extra[f"sweep_u3_q{tq}_c128"] = {
+                "kernel_gbs": sres["kernel_gbs"], "frac_of_peak": sres["kernel_gbs"] / HBM_PEAK_GBS,
+                "kernel_ms": sres["kernel_ms_avg"]}
+            log(f"bench: single-gate sweep q{tq} c128 {sres['kernel_gbs']:.0f} GB/s "
+                f"({100 * sres['kernel_gbs'] / HBM_PEAK_GBS:.1f} % of peak)")
         adj = run_adjoint(24, 12, args.adjoint_batch, 16, seed)
         log(f"bench: adjoint 24q {adj['grads_per_s']:.0f} grads/s, param shift "
             f"{adj['param_shift_grads_per_s']:.1f} grads/s ({adj['speedup_vs_param_shift']:.0f}x)")
