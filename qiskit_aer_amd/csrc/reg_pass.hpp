@@ -606,6 +606,20 @@ __device__ __forceinline__ Amps<T, NR> pass_through(const Amps<T, NR> &a) {
   return o;
 }
 
+// Amplitude loads and stores of the HBM sweep, marked non-temporal: each
+// amplitude is read and written once per pass, so caching it gains nothing.
+template <typename T>
+__device__ __forceinline__ cx<T> ld_stream(const cx<T> *p) {
+  typedef T v2 __attribute__((ext_vector_type(2)));
+  const v2 v = __builtin_nontemporal_load(reinterpret_cast<const v2 *>(p));
+  return mk<T>(v.x, v.y);
+}
+template <typename T>
+__device__ __forceinline__ void st_stream(cx<T> *p, cx<T> a) {
+  typedef T v2 __attribute__((ext_vector_type(2)));
+  __builtin_nontemporal_store(v2{a.x, a.y}, reinterpret_cast<v2 *>(p));
+}
+
 // A wave-uniform 64-bit value moved to scalar registers.
 __device__ __forceinline__ uint64_t uniform64(uint64_t v) {
   const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v));
@@ -885,7 +899,7 @@ __device__ __forceinline__ void reg_tile(cx<T> *__restrict__ sv, const PassDesc 
       const RegOff<NR> ro = reg_offset<R>(e.u.lay);
       const cx<T> *p = sv + base + thread_offset<TB>(e.u.lay, tid);
 #pragma unroll
-      for (int j = 0; j < NR; ++j) a.v[j] = p[ro.gr[j]];
+      for (int j = 0; j < NR; ++j) a.v[j] = ld_stream(p + ro.gr[j]);
     } else {
 #pragma unroll
       for (int j = 0; j < NR; ++j) a.v[j] = lds[st.tl | st.ri.at(j)];
@@ -912,7 +926,7 @@ __device__ __forceinline__ void reg_tile(cx<T> *__restrict__ sv, const PassDesc 
       const RegOff<NR> ro = reg_offset_mapped<R>(l.u.lay, st.ri);
       cx<T> *p = sv + base + thread_offset<TB>(l.u.lay, tid);
 #pragma unroll
-      for (int j = 0; j < NR; ++j) p[ro.gr[j]] = a.v[j];
+      for (int j = 0; j < NR; ++j) st_stream(p + ro.gr[j], a.v[j]);
       return;
     }
     // SPILL: the tile goes to LDS in natural order for the LDS primitives.
