@@ -265,6 +265,13 @@ constexpr int top_bit() {
   return m >= 16 ? 4 : m >= 8 ? 3 : m >= 4 ? 2 : m >= 2 ? 1 : 0;
 }
 
+// Index of the pair whose lower register is j (j has bit hb clear): j with
+// bit hb removed.
+template <int hb>
+constexpr int k_of(int j) {
+  return ((j >> (hb + 1)) << hb) | (j & ((1 << hb) - 1));
+}
+
 template <typename T, int NR, int m>
 __device__ __forceinline__ Amps<T, NR> q_u1m(const Amps<T, NR> &a, uint32_t roles, cx<T> m0,
                                              cx<T> m1, cx<T> m2, cx<T> m3) {
@@ -275,8 +282,31 @@ __device__ __forceinline__ Amps<T, NR> q_u1m(const Amps<T, NR> &a, uint32_t role
   for (int j = 0; j < NR; ++j) {
     if (j & (1 << hb)) continue;
     const int jj = j ^ m;
-    const bool sw = (roles >> k) & 1u;
-    const cx<T> c00 = sw ? m3 : m0, c01 = sw ? m1 : m2, c10 = sw ? m2 : m1, c11 = sw ? m0 : m3;
+    cx<T> c00, c01, c10, c11;
+    if constexpr (sizeof(T) == 8) {
+      // one bit test and eight 64-bit scalar selects (the compiler splits
+      // 64-bit selects into two 32-bit ones)
+      asm("s_bitcmp1_b32 %[r], %[k]\n\t"
+          "s_cselect_b64 %[c00x], %[m3x], %[m0x]\n\t"
+          "s_cselect_b64 %[c00y], %[m3y], %[m0y]\n\t"
+          "s_cselect_b64 %[c01x], %[m1x], %[m2x]\n\t"
+          "s_cselect_b64 %[c01y], %[m1y], %[m2y]\n\t"
+          "s_cselect_b64 %[c10x], %[m2x], %[m1x]\n\t"
+          "s_cselect_b64 %[c10y], %[m2y], %[m1y]\n\t"
+          "s_cselect_b64 %[c11x], %[m0x], %[m3x]\n\t"
+          "s_cselect_b64 %[c11y], %[m0y], %[m3y]"
+          : [c00x] "=&s"(c00.x), [c00y] "=&s"(c00.y), [c01x] "=&s"(c01.x), [c01y] "=&s"(c01.y),
+            [c10x] "=&s"(c10.x), [c10y] "=&s"(c10.y), [c11x] "=&s"(c11.x), [c11y] "=&s"(c11.y)
+          : [r] "s"(roles), [k] "n"(k_of<hb>(j)), [m0x] "s"(m0.x), [m0y] "s"(m0.y), [m1x] "s"(m1.x),
+            [m1y] "s"(m1.y), [m2x] "s"(m2.x), [m2y] "s"(m2.y), [m3x] "s"(m3.x), [m3y] "s"(m3.y)
+          : "scc");
+    } else {
+      const bool sw = (roles >> k) & 1u;
+      c00 = sw ? m3 : m0;
+      c01 = sw ? m1 : m2;
+      c10 = sw ? m2 : m1;
+      c11 = sw ? m0 : m3;
+    }
     o.v[j] = mv2(c00, a.v[j], c01, a.v[jj]);
     o.v[jj] = mv2(c10, a.v[j], c11, a.v[jj]);
     ++k;
