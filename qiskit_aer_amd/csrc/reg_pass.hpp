@@ -663,17 +663,21 @@ __device__ __forceinline__ uint32_t opaque_u32(uint32_t v) {
   return v;
 }
 
-// The head of an op, read one op ahead: its scalar fields; for complex128
-// in the lean variant (PF), the 2x2 matrix of the payload as well (scalar
-// loads, so the matrix is in SGPRs when the op starts; the full variant has
-// no SGPRs to spare for it); for complex64, the lane's row of the real 4x4
+// The head of an op, read one op ahead: its scalar fields; in the lean
+// variant (PF) also the controls and the first words of the payload (for
+// complex128 the 2x2 matrix, for complex64 the diagonal or phase), as scalar
+// loads, so that they are in SGPRs when the op starts (the full variant has
+// no SGPRs to spare for them); for complex64, the lane's row of the real 4x4
 // matrix that the MFMA paths multiply by (row lane % 4, a per-lane vector
 // load).
 template <typename T, bool PF>
 struct OpHead {
   uint32_t kind, a, b, extra;
+  uint32_t cl, cvl;
+  uint64_t cg, cvg;
   f32x4 arow;
   double m[8];
+  float mf[4];
 };
 
 template <bool PF, typename T>
@@ -683,16 +687,55 @@ __device__ __forceinline__ OpHead<T, PF> load_head(const ROp<T> &op) {
   h.a = op.a;
   h.b = op.b;
   h.extra = op.extra;
+  h.cl = h.cvl = 0;
+  h.cg = h.cvg = 0;
   h.arow = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int e = 0; e < 8; ++e) h.m[e] = 0.0;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) h.mf[e] = 0.f;
+  if constexpr (PF) {
+    h.cl = op.cl;
+    h.cvl = op.cvl;
+    h.cg = op.cg;
+    h.cvg = op.cvg;
+  }
   if constexpr (sizeof(T) == 4) {
     h.arow = *reinterpret_cast<const f32x4 *>(op.u.mr + 4 * (threadIdx.x & 3u));
+    if constexpr (PF) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) h.mf[e] = op.u.mr[e];
+    }
   } else if constexpr (PF) {
 #pragma unroll
     for (int e = 0; e < 8; ++e) h.m[e] = op.u.m[e];
   }
   return h;
+}
+
+// The op's controls and its first payload words, from the head where it
+// holds them.
+template <typename T, bool PF>
+__device__ __forceinline__ uint32_t op_cl(const ROp<T> &op, const OpHead<T, PF> &hd) {
+  return PF ? hd.cl : op.cl;
+}
+template <typename T, bool PF>
+__device__ __forceinline__ uint32_t op_cvl(const ROp<T> &op, const OpHead<T, PF> &hd) {
+  return PF ? hd.cvl : op.cvl;
+}
+template <typename T, bool PF>
+__device__ __forceinline__ bool op_tile_ok(const ROp<T> &op, const OpHead<T, PF> &hd, uint64_t base) {
+  return PF ? (base & hd.cg) == hd.cvg : (base & op.cg) == op.cvg;
+}
+template <typename T, bool PF>
+__device__ __forceinline__ cx<T> op_m(const ROp<T> &op, const OpHead<T, PF> &hd, int e) {
+  if constexpr (PF) {
+    if constexpr (sizeof(T) == 4)
+      return mk<T>(hd.mf[2 * e], hd.mf[2 * e + 1]);
+    else
+      return mk<T>(hd.m[2 * e], hd.m[2 * e + 1]);
+  }
+  return mk<T>(op.u.m[2 * e], op.u.m[2 * e + 1]);
 }
 
 // A register op that rewrites every amplitude (a layout change or a gate):
@@ -760,7 +803,7 @@ __device__ __forceinline__ Amps<T, 1 << R> rw_op(const ROp<T> &op, const OpHead<
     }
   }
   if (kind == Q_U1X) {
-    const bool fx = (base & op.cg) == op.cvg;
+    const bool fx = op_tile_ok(op, hd, base);
     if constexpr (sizeof(T) == 4) {
       // A operand of issue k: row (lane % 4), column k of the real 4x4
       // matrix; U X swaps the columns that act on a0 and a1 (k and k ^ 2).
@@ -783,7 +826,7 @@ __device__ __forceinline__ Amps<T, 1 << R> rw_op(const ROp<T> &op, const OpHead<
     }
   }
   if (kind == Q_U1MX) {  // the same, with an X folded in where its tile predicate holds
-    const bool fx = (base & op.cg) == op.cvg;
+    const bool fx = op_tile_ok(op, hd, base);
     if constexpr (sizeof(T) == 4 && !FULL) {
       const f32x4 An = fx ? f32x4{hd.arow.z, hd.arow.w, hd.arow.x, hd.arow.y} : hd.arow;
       return d_u1m_mfma<NR>(ra, extra, a, An, mfma_row_swapped(An));
@@ -814,9 +857,9 @@ __device__ __forceinline__ Amps<T, 1 << R> rw_op(const ROp<T> &op, const OpHead<
     for (int j = 0; j < NR; ++j) o.v[j] = lds[st.tl | st.ri.at(j)];
     return o;
   }
-  const bool tile_ok = (base & op.cg) == op.cvg;
-  const uint32_t cl = tile_ok ? op.cl : 0u;
-  const uint32_t cvl = tile_ok ? op.cvl : 1u;
+  const bool tile_ok = op_tile_ok(op, hd, base);
+  const uint32_t cl = tile_ok ? op_cl(op, hd) : 0u;
+  const uint32_t cvl = tile_ok ? op_cvl(op, hd) : 1u;
   const uint32_t tlc = st.tl & cl;
   if (FULL && kind == Q_U1C) {
     const cx<T> m0 = mat_at<T>(op.u.m, 0), m1 = mat_at<T>(op.u.m, 1), m2 = mat_at<T>(op.u.m, 2), m3 = mat_at<T>(op.u.m, 3);
@@ -846,7 +889,7 @@ __device__ __forceinline__ Amps<T, 1 << R> rw_op(const ROp<T> &op, const OpHead<
     const uint32_t clr = cl & rbits, clt = cl & ~rbits;
     const bool lane_ok = (st.tl & clt) == (cvl & clt);
     if (kind == Q_D1) {
-      const cx<T> d0 = mat_at<T>(op.u.m, 0), d1 = mat_at<T>(op.u.m, 1);
+      const cx<T> d0 = op_m(op, hd, 0), d1 = op_m(op, hd, 1);
       const uint32_t t = ra;
       if (cl == 0) {
         if ((rbits >> t) & 1u) {
@@ -869,7 +912,7 @@ __device__ __forceinline__ Amps<T, 1 << R> rw_op(const ROp<T> &op, const OpHead<
       return o;
     }
     // Q_PHASE
-    const cx<T> ph = mat_at<T>(op.u.m, 0);
+    const cx<T> ph = op_m(op, hd, 0);
 #pragma unroll
     for (int j = 0; j < NR; ++j) o.v[j] = cmul(a.v[j], csel((tlc | (st.ri.at(j) & cl)) == cvl, ph, one));
     return o;
