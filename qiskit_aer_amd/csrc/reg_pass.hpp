@@ -282,33 +282,23 @@ __device__ __forceinline__ Amps<T, NR> q_u1m(const Amps<T, NR> &a, uint32_t role
   for (int j = 0; j < NR; ++j) {
     if (j & (1 << hb)) continue;
     const int jj = j ^ m;
-    cx<T> c00, c01, c10, c11;
     if constexpr (sizeof(T) == 8) {
-      // one bit test and eight 64-bit scalar selects (the compiler splits
-      // 64-bit selects into two 32-bit ones)
-      asm("s_bitcmp1_b32 %[r], %[k]\n\t"
-          "s_cselect_b64 %[c00x], %[m3x], %[m0x]\n\t"
-          "s_cselect_b64 %[c00y], %[m3y], %[m0y]\n\t"
-          "s_cselect_b64 %[c01x], %[m1x], %[m2x]\n\t"
-          "s_cselect_b64 %[c01y], %[m1y], %[m2y]\n\t"
-          "s_cselect_b64 %[c10x], %[m2x], %[m1x]\n\t"
-          "s_cselect_b64 %[c10y], %[m2y], %[m1y]\n\t"
-          "s_cselect_b64 %[c11x], %[m0x], %[m3x]\n\t"
-          "s_cselect_b64 %[c11y], %[m0y], %[m3y]"
-          : [c00x] "=&s"(c00.x), [c00y] "=&s"(c00.y), [c01x] "=&s"(c01.x), [c01y] "=&s"(c01.y),
-            [c10x] "=&s"(c10.x), [c10y] "=&s"(c10.y), [c11x] "=&s"(c11.x), [c11y] "=&s"(c11.y)
-          : [r] "s"(roles), [k] "n"(k_of<hb>(j)), [m0x] "s"(m0.x), [m0y] "s"(m0.y), [m1x] "s"(m1.x),
-            [m1y] "s"(m1.y), [m2x] "s"(m2.x), [m2y] "s"(m2.y), [m3x] "s"(m3.x), [m3y] "s"(m3.y)
-          : "scc");
+      // a uniform branch on the pair's role: no per-pair selects
+      if ((roles >> k_of<hb>(j)) & 1u) {
+        asm volatile("");  // keeps the branch (no if-conversion into selects)
+        o.v[j] = mv2(m3, a.v[j], m1, a.v[jj]);
+        o.v[jj] = mv2(m2, a.v[j], m0, a.v[jj]);
+      } else {
+        o.v[j] = mv2(m0, a.v[j], m2, a.v[jj]);
+        o.v[jj] = mv2(m1, a.v[j], m3, a.v[jj]);
+      }
     } else {
       const bool sw = (roles >> k) & 1u;
-      c00 = sw ? m3 : m0;
-      c01 = sw ? m1 : m2;
-      c10 = sw ? m2 : m1;
-      c11 = sw ? m0 : m3;
+      const cx<T> c00 = sw ? m3 : m0, c01 = sw ? m1 : m2;
+      const cx<T> c10 = sw ? m2 : m1, c11 = sw ? m0 : m3;
+      o.v[j] = mv2(c00, a.v[j], c01, a.v[jj]);
+      o.v[jj] = mv2(c10, a.v[j], c11, a.v[jj]);
     }
-    o.v[j] = mv2(c00, a.v[j], c01, a.v[jj]);
-    o.v[jj] = mv2(c10, a.v[j], c11, a.v[jj]);
     ++k;
   }
   return o;
@@ -895,7 +885,22 @@ __device__ __forceinline__ Amps<T, 1 << R> rw_op(const ROp<T> &op, const OpHead<
         if ((rbits >> t) & 1u) {
           // target on a register bit: the factor of each register is uniform
 #pragma unroll
-          for (int j = 0; j < NR; ++j) o.v[j] = cmul(a.v[j], ((st.ri.at(j) >> t) & 1u) ? d1 : d0);
+          for (int j = 0; j < NR; ++j) {
+            if constexpr (sizeof(T) == 8) {
+              // one bit test and two 64-bit scalar selects
+              cx<T> d;
+              asm("s_bitcmp1_b32 %[v], %[t]\n\t"
+                  "s_cselect_b64 %[dx], %[d1x], %[d0x]\n\t"
+                  "s_cselect_b64 %[dy], %[d1y], %[d0y]"
+                  : [dx] "=&s"(d.x), [dy] "=&s"(d.y)
+                  : [v] "s"(st.ri.at(j)), [t] "s"(t), [d0x] "s"(d0.x), [d0y] "s"(d0.y), [d1x] "s"(d1.x),
+                    [d1y] "s"(d1.y)
+                  : "scc");
+              o.v[j] = cmul(a.v[j], d);
+            } else {
+              o.v[j] = cmul(a.v[j], ((st.ri.at(j) >> t) & 1u) ? d1 : d0);
+            }
+          }
         } else {
           const cx<T> dl = csel(((st.tl >> t) & 1u) != 0, d1, d0);
 #pragma unroll
@@ -913,8 +918,25 @@ __device__ __forceinline__ Amps<T, 1 << R> rw_op(const ROp<T> &op, const OpHead<
     }
     // Q_PHASE
     const cx<T> ph = op_m(op, hd, 0);
+    if constexpr (sizeof(T) == 8) {
+      // The lane part of the predicate selects the factor once; the register
+      // part is uniform per register: a branch (registers it excludes are
+      // only copied).
+      const cx<T> f = csel(lane_ok, ph, one);
+      const uint32_t cvr = cvl & clr;
 #pragma unroll
-    for (int j = 0; j < NR; ++j) o.v[j] = cmul(a.v[j], csel((tlc | (st.ri.at(j) & cl)) == cvl, ph, one));
+      for (int j = 0; j < NR; ++j) {
+        if ((st.ri.at(j) & clr) == cvr) {
+          asm volatile("");  // keeps the branch (no if-conversion into selects)
+          o.v[j] = cmul(a.v[j], f);
+        } else {
+          o.v[j] = a.v[j];
+        }
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < NR; ++j) o.v[j] = cmul(a.v[j], csel((tlc | (st.ri.at(j) & cl)) == cvl, ph, one));
+    }
     return o;
   }
   if constexpr (!FULL) return pass_through(a);  // not reached: lean programs hold no other kinds
