@@ -895,6 +895,9 @@ void register_program(const std::vector<TileOp> &in, uint32_t B, uint32_t R, uin
       if (x & (1u << q)) y ^= colinv[q];
     return y;
   };
+  // Tile bits of register j under the relabelling in force (the device's
+  // RegIdx::at(j)).
+  auto reg_tile_bits = [&](uint32_t j) { return tile_mask(cinv ^ lin(j)); };
   auto phys_of = [&](uint32_t y) {  // the physical mask x with lin(x) == y
     for (uint32_t x = 0; x < (1u << R); ++x)
       if (lin(x) == y) return x;
@@ -1086,6 +1089,9 @@ void register_program(const std::vector<TileOp> &in, uint32_t B, uint32_t R, uin
         if (op.k == 1 && op.pos[0] != kNotLocal) {
           r.kind = Q_D1;
           r.a = op.pos[0];
+          r.b = 0;  // registers whose target bit is 1 (a register target)
+          for (uint32_t j = 0; j < (1u << R); ++j)
+            if ((reg_tile_bits(j) >> op.pos[0]) & 1u) r.b |= 1u << j;
           for (int e = 0; e < 2; ++e) {
             r.u.m[2 * e] = pool[op.off + e].real();
             r.u.m[2 * e + 1] = pool[op.off + e].imag();
@@ -1099,6 +1105,9 @@ void register_program(const std::vector<TileOp> &in, uint32_t B, uint32_t R, uin
       case K_MCPHASE:
         materialize(reg_bits(op.cl));
         r.kind = Q_PHASE;
+        r.b = 0;  // registers that meet the register part of the predicate
+        for (uint32_t j = 0; j < (1u << R); ++j)
+          if ((reg_tile_bits(j) & op.cl & cur_rs) == (op.cvl & op.cl & cur_rs)) r.b |= 1u << j;
         r.u.m[0] = op.pr;
         r.u.m[1] = op.pi;
         break;

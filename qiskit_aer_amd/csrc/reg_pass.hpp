@@ -39,8 +39,10 @@ enum RegPassOpKind : uint32_t {
   Q_X = 48,         // X on register bit a where (l & cl) == cvl
   Q_Y = 49,         // Y on register bit a where (l & cl) == cvl
   Q_U2 = 50,        // 4x4 matrix pool[extra..] on register bits a (matrix bit 0), b (bit 1)
-  Q_D1 = 51,        // diag(u.m[0..1], u.m[2..3]) on local bit a where (l & cl) == cvl
-  Q_PHASE = 52,     // multiply by u.m[0..1] where (l & cl) == cvl
+  Q_D1 = 51,        // diag(u.m[0..1], u.m[2..3]) on local bit a where (l & cl) == cvl; bit j
+                    // of b: tile bit a of register j is 1
+  Q_PHASE = 52,     // multiply by u.m[0..1] where (l & cl) == cvl; bit j of b: register j
+                    // meets the register part of the predicate
   Q_DIAG = 53,      // diagonal of TileOp extra (local and fixed bits)
   Q_PROBE = 54,     // Pauli probe whose X/Y bits are register bits (u.pr)
   Q_NOP = 55,       // keeps the amplitudes (pads a register phase to an even length)
@@ -887,18 +889,18 @@ __device__ __forceinline__ Amps<T, 1 << R> rw_op(const ROp<T> &op, const OpHead<
 #pragma unroll
           for (int j = 0; j < NR; ++j) {
             if constexpr (sizeof(T) == 8) {
-              // one bit test and two 64-bit scalar selects
+              // one test of the register's bit in b and two 64-bit scalar selects
               cx<T> d;
-              asm("s_bitcmp1_b32 %[v], %[t]\n\t"
+              asm("s_bitcmp1_b32 %[v], %[j]\n\t"
                   "s_cselect_b64 %[dx], %[d1x], %[d0x]\n\t"
                   "s_cselect_b64 %[dy], %[d1y], %[d0y]"
                   : [dx] "=&s"(d.x), [dy] "=&s"(d.y)
-                  : [v] "s"(st.ri.at(j)), [t] "s"(t), [d0x] "s"(d0.x), [d0y] "s"(d0.y), [d1x] "s"(d1.x),
+                  : [v] "s"(rb), [j] "n"(j), [d0x] "s"(d0.x), [d0y] "s"(d0.y), [d1x] "s"(d1.x),
                     [d1y] "s"(d1.y)
                   : "scc");
               o.v[j] = cmul(a.v[j], d);
             } else {
-              o.v[j] = cmul(a.v[j], ((st.ri.at(j) >> t) & 1u) ? d1 : d0);
+              o.v[j] = cmul(a.v[j], ((rb >> j) & 1u) ? d1 : d0);
             }
           }
         } else {
@@ -920,13 +922,12 @@ __device__ __forceinline__ Amps<T, 1 << R> rw_op(const ROp<T> &op, const OpHead<
     const cx<T> ph = op_m(op, hd, 0);
     if constexpr (sizeof(T) == 8) {
       // The lane part of the predicate selects the factor once; the register
-      // part is uniform per register: a branch (registers it excludes are
-      // only copied).
+      // part, uniform per register, comes as a mask (bit j of b): a branch
+      // (registers it excludes are only copied).
       const cx<T> f = csel(lane_ok, ph, one);
-      const uint32_t cvr = cvl & clr;
 #pragma unroll
       for (int j = 0; j < NR; ++j) {
-        if ((st.ri.at(j) & clr) == cvr) {
+        if ((rb >> j) & 1u) {
           asm volatile("");  // keeps the branch (no if-conversion into selects)
           o.v[j] = cmul(a.v[j], f);
         } else {
