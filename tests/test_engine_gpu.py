@@ -378,3 +378,40 @@ def test_register_relabelling_random_mix(precision, seed):
         want = [o.expval_pauli(near, lab) for lab in labels]
         assert np.max(np.abs(np.array(got) - np.array(want))) <= 10 * TOL[precision]
     close(e, o, precision)
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_phase_and_diagonal_register_masks(precision):
+    """Phases and single-qubit diagonals under register relabelling, with
+    predicates that mix register bits, thread bits and qubits fixed by the
+    tile: each such op carries a register mask that the host computes from
+    the relabelling in force."""
+    n = 16
+    print(f"case: phase/diagonal register masks {precision}", flush=True)
+    e, o, rng = pair(n, 900, precision)
+    near = [0, 1, 2, 3, 4, 5, 6, 7]
+    far = [11, 13, 15]
+    for _ in range(300):
+        kind = rng.integers(0, 8)
+        a, b = (int(x) for x in rng.choice(near, 2, replace=False))
+        if kind <= 2:
+            e.apply_mcx([a, b]); o.apply_mcx([a, b])
+        elif kind == 3:
+            e.apply_mcx([b]); o.apply_mcx([b])
+        elif kind == 4:
+            k = int(rng.integers(1, 4))
+            qs = [int(x) for x in rng.choice(near + far, k, replace=False)]
+            ph = np.exp(1j * rng.uniform(-3, 3))
+            e.apply_mcphase(qs, ph); o.apply_mcphase(qs, ph)
+        elif kind == 5:
+            q = int(rng.choice(near + far))
+            t = rng.uniform(-3, 3)
+            e.apply_rotation([q], "z", t); o.apply_rotation([q], "z", t)
+        elif kind == 6:
+            d = np.exp(1j * rng.uniform(-3, 3, 2))
+            e.apply_diagonal_matrix([a], d); o.apply_diagonal_matrix([a], d)
+        else:
+            m = D.random_unitary(2, rng)
+            e.apply_matrix([a], m); o.apply_matrix([a], m)
+    close(e, o, precision)
+
