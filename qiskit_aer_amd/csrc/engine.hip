@@ -1385,9 +1385,19 @@ void Engine::emit_program(Program &prog) {
 }
 
 namespace {
+// Register passes of at most this many ops (LOAD, STORE and a few gates)
+// launch the wide lean variant.
+constexpr uint32_t kWidePassOps = 8;
+
 template <typename T, int B, int R>
-void launch_reg_pass(bool lean, uint32_t grid, hipStream_t stream, cx<T> *sv, const PassDesc *pd,
+void launch_reg_pass(bool lean, bool wide, uint32_t grid, hipStream_t stream, cx<T> *sv, const PassDesc *pd,
                      const ROp<T> *rops, const TileOp *tops, const cx<double> *pool, double *probes) {
+  if constexpr (sizeof(T) == 8 && R == 3) {
+    if (lean && wide) {
+      k_reg_pass<T, B, R, false, true><<<grid, 1 << (B - R), 0, stream>>>(sv, pd, rops, tops, pool, probes);
+      return;
+    }
+  }
   if (lean)
     k_reg_pass<T, B, R, false><<<grid, 1 << (B - R), 0, stream>>>(sv, pd, rops, tops, pool, probes);
   else
@@ -1466,20 +1476,22 @@ void Engine::flush() {
       const uint32_t grid = static_cast<uint32_t>(1ull << pd.ncomp);
       bool lean = true;
       for (uint32_t k = 0; k < pd.nops && lean; ++k) lean = lean_kind(rops[pd.op_begin + k].kind);
+      // short passes (a few gates) are an HBM sweep: more waves per SIMD
+      const bool wide = pd.nops <= kWidePassOps;
       if (prec_ == AERHIP_PRECISION_DOUBLE) {
         auto *sv = static_cast<cx<double> *>(sv_);
         auto *ro = static_cast<const ROp<double> *>(drops);
         if (reg_bits_ == 3)
-          launch_reg_pass<double, kTileBitsC128, 3>(lean, grid, stream_, sv, dpass + i, ro, dops, dpool, dprobes);
+          launch_reg_pass<double, kTileBitsC128, 3>(lean, wide, grid, stream_, sv, dpass + i, ro, dops, dpool, dprobes);
         else
-          launch_reg_pass<double, kTileBitsC128, 4>(lean, grid, stream_, sv, dpass + i, ro, dops, dpool, dprobes);
+          launch_reg_pass<double, kTileBitsC128, 4>(lean, wide, grid, stream_, sv, dpass + i, ro, dops, dpool, dprobes);
       } else {
         auto *sv = static_cast<cx<float> *>(sv_);
         auto *ro = static_cast<const ROp<float> *>(drops);
         if (reg_bits_ == 4)
-          launch_reg_pass<float, kTileBitsC64, 4>(lean, grid, stream_, sv, dpass + i, ro, dops, dpool, dprobes);
+          launch_reg_pass<float, kTileBitsC64, 4>(lean, wide, grid, stream_, sv, dpass + i, ro, dops, dpool, dprobes);
         else
-          launch_reg_pass<float, kTileBitsC64, 5>(lean, grid, stream_, sv, dpass + i, ro, dops, dpool, dprobes);
+          launch_reg_pass<float, kTileBitsC64, 5>(lean, wide, grid, stream_, sv, dpass + i, ro, dops, dpool, dprobes);
       }
     } else if (pd.kind == 3) {
       // probe sweep over the whole state, one workgroup per tile-sized slice

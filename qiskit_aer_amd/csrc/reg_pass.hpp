@@ -1042,10 +1042,13 @@ __device__ __forceinline__ void reg_tile(cx<T> *__restrict__ sv, const PassDesc 
 
 // Waves per SIMD that the tile's LDS allows (160 KiB per CU, four SIMDs;
 // 1 KiB per workgroup for the small arrays beside the tile); register
-// allocation is held to that occupancy.
-template <typename T, int B, int R>
+// allocation is held to that occupancy.  WIDE: the lean variant, which uses
+// no small arrays, at five 32 KiB tiles per CU (its complex128 registers fit
+// the 96 VGPRs of five waves); launched for short passes, whose time is the
+// HBM sweep.
+template <typename T, int B, int R, bool WIDE = false>
 constexpr int reg_pass_waves() {
-  const int blocks = (160 * 1024) / ((static_cast<int>(sizeof(cx<T>)) << B) + 1024);
+  const int blocks = (160 * 1024) / ((static_cast<int>(sizeof(cx<T>)) << B) + (WIDE ? 0 : 1024));
   const int waves = blocks * ((1 << (B - R)) / 64) / 4;
   return waves < 1 ? 1 : waves > 8 ? 8 : waves;
 }
@@ -1053,12 +1056,13 @@ constexpr int reg_pass_waves() {
 // One workgroup per tile.  FULL = false: a lean variant for programs whose
 // ops are all LOAD, STORE, RELAYOUT, U1, U1M, X, D1, PHASE and NOP (a single
 // register phase); the smaller op loop keeps the frequent gates faster.
-template <typename T, int B, int R, bool FULL>
+template <typename T, int B, int R, bool FULL, bool WIDE = false>
 __global__ __launch_bounds__(1 << (B - R))
-__attribute__((amdgpu_waves_per_eu(reg_pass_waves<T, B, R>()))) void k_reg_pass(
+__attribute__((amdgpu_waves_per_eu(reg_pass_waves<T, B, R, WIDE>()))) void k_reg_pass(
     cx<T> *__restrict__ sv, const PassDesc *__restrict__ pd, const ROp<T> *__restrict__ rops,
     const TileOp *__restrict__ tops, const cx<double> *__restrict__ pool,
     double *__restrict__ probes) {
+  static_assert(!WIDE || (!FULL && sizeof(T) == 8), "the wide variant is the lean complex128 one");
   __shared__ cx<T> lds[1 << B];
   __shared__ cx<T> mat[16];
   __shared__ uint32_t doff[32];
