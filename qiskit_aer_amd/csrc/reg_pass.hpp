@@ -839,14 +839,22 @@ __device__ __forceinline__ Amps<T, 1 << R> rw_op(const ROp<T> &op, const OpHead<
     return pass_through(a);
   }
   if (kind == Q_RELAYOUT) {
+    // complex64: the tile passes through LDS in a swizzled order, slot x at
+    // x ^ ((x >> 4) & 15), so that consecutive lanes reach distinct banks also
+    // when the register bits are the low tile bits.  The swizzle is linear and
+    // the thread and register parts of x are disjoint, so it splits into a
+    // per-lane and a per-register term.
+    auto swz = [](uint32_t x) { return sizeof(T) == 4 ? x ^ ((x >> 4) & 15u) : x; };
     __syncthreads();
+    const uint32_t tw = swz(st.tl);
 #pragma unroll
-    for (int j = 0; j < NR; ++j) lds[st.tl | st.ri.at(j)] = a.v[j];
+    for (int j = 0; j < NR; ++j) lds[tw ^ swz(st.ri.at(j))] = a.v[j];
     __syncthreads();
     st = make_state<R, TB>(op.u.lay, tid);
+    const uint32_t tr = swz(st.tl);
     Amps<T, NR> o;
 #pragma unroll
-    for (int j = 0; j < NR; ++j) o.v[j] = lds[st.tl | st.ri.at(j)];
+    for (int j = 0; j < NR; ++j) o.v[j] = lds[tr ^ swz(st.ri.at(j))];
     return o;
   }
   const bool tile_ok = op_tile_ok(op, hd, base);
