@@ -29,9 +29,9 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X published FP64 peak (vector and matrix)
 
 # HBM bytes per tile-pass launch of the default workload (30 qubits, c128),
 # from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (profiles/reg_pass_30q_brickwork_c128.md):
-# 2.0 x 8 388 781 KiB fetched (FETCH_SIZE counts half the bytes, calibrated on a
-# kernel that reads the 16 GiB state once) + 16 777 455 KiB written.
-MEASURED_TRAFFIC_30Q_C128 = (2 * 8388781 + 16777455) * 1024
+# 2.0 x 8 388 793 KiB fetched (FETCH_SIZE counts half the bytes, calibrated on a
+# kernel that reads the 16 GiB state once) + 16 777 476 KiB written.
+MEASURED_TRAFFIC_30Q_C128 = (2 * 8388793 + 16777476) * 1024
 
 
 def log(msg: str) -> None:
