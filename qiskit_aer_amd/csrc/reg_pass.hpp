@@ -803,8 +803,14 @@ __device__ __forceinline__ Amps<T, 1 << R> rw_op(const ROp<T> &op, const OpHead<
       const float A2 = fx ? hd.arow.x : hd.arow.z, A3 = fx ? hd.arow.y : hd.arow.w;
       return d_u1_mfma<NR>(ra, a, A0, A1, A2, A3);
     } else {
+      // a uniform branch on the tile predicate: the folded X swaps the
+      // matrix columns (no selects)
       cx<T> m0, m1, m2, m3;
-      u1_matrix(op, hd, fx, m0, m1, m2, m3);
+      u1_matrix(op, hd, false, m0, m1, m2, m3);
+      if (fx) {
+        asm volatile("");  // keeps the branch (no if-conversion into selects)
+        return d_u1<T, NR>(ra, a, m2, m3, m0, m1);
+      }
       return d_u1<T, NR>(ra, a, m0, m1, m2, m3);
     }
   }
