@@ -207,16 +207,28 @@ def run_batched(num_circuits: int, terms: int, seed: int) -> dict:
 
 
 def cpu_baseline(n: int, layers: int, seed: int) -> dict:
-    """The oracle (restated reference CPU path, fusion on, OpenMP on every
-    host core) on the first `layers` layers of the same brickwork."""
+    """The reference's CPU path on the first `layers` layers of the same
+    brickwork: the oracle's restated driver and Aer fusion (max 5 qubits, the
+    AVX2 cost table) with every fused matrix and diagonal applied by the
+    reference's own compiled Transformer / TransformerAVX2 (oracle/_ref, picked
+    by the reference's CPUID check), OpenMP on the host cores."""
     from oracle import pyoracle as O
     from qiskit_aer_amd import circuits
     c = circuits.brickwork_u3_cx(n, layers, seed, measure=False)
     threads = len(os.sched_getaffinity(0))
-    _, _, timing = O.run_circuit(c, shots=0, seed=0, threads=threads, statevector=False)
-    return {"value": c.count_gates() / timing[0], "unit": "gates/s", "cores": threads, "kind": "port",
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
+    kind, path = "port", "restated generic transformer"
+    if O.ref() is not None:
+        with O.reference_transformer() as rt:
+            _, _, timing = O.run_circuit(c, shots=0, seed=0, threads=threads, statevector=False)
+        kind = "reference"
+        path = "reference TransformerAVX2" if rt.avx2 else "reference Transformer (no AVX2 on this host)"
+    else:
+        _, _, timing = O.run_circuit(c, shots=0, seed=0, threads=threads, statevector=False)
+    return {"value": c.count_gates() / timing[0], "unit": "gates/s", "cores": threads, "kind": kind,
             "sample": f"{n}q c128 brickwork, first {layers} of 40 layers ({c.count_gates()} gates), "
-                      f"Aer fusion on (max 5), gate phase {timing[0]:.2f} s"}
+                      f"Aer fusion on (max 5), {path}, gate phase {timing[0]:.2f} s"}
 
 
 def profile_summary(trace_db: str, fetch_db: str = None, write_db: str = None) -> str:
@@ -335,7 +347,7 @@ def main() -> int:
     cpu = None
     if not args.no_cpu:
         log("bench: cpu baseline")
-        cpu = cpu_baseline(n, 2, seed)
+        cpu = cpu_baseline(n, 4, seed)
         log(f"bench: cpu {cpu['value']:.2f} gates/s on {cpu['cores']} cores")
 
     line = {

@@ -135,6 +135,21 @@ int aerhip_qv_set_tile_bits(aerhip_qv *qv, int tile_bits, int low_bits);
  * register layout changes inside tile passes}. */
 int aerhip_qv_stats(aerhip_qv *qv, uint64_t *out4);
 
+/* The engine's fusion, configured by Aer's options fusion_enable,
+ * fusion_max_qubit and fusion_threshold (src/transpile/fusion.hpp:757-763,
+ * 873).  Inside each tile pass the engine merges runs of diagonal gates
+ * (phase gates on one or two qubits, single-qubit diagonals) into one
+ * table-driven diagonal op, as Aer's DiagonalFusion (fusion.hpp:520-726)
+ * merges them into one diagonal matrix.  enable = 0 turns it off; a fused op
+ * holds at most max_qubit qubits (>= 2); fusion applies only to states of
+ * more than `threshold` qubits.  Handles start enabled, max_qubit 64,
+ * threshold 0; aerhip_run_circuit passes the run configuration's values. */
+int aerhip_qv_set_fusion(aerhip_qv *qv, int enable, int max_qubit, int threshold);
+
+/* No counterpart in the reference: number of gate primitives merged into
+ * fused ops on this handle so far. */
+int aerhip_qv_fused_count(aerhip_qv *qv, uint64_t *out);
+
 /* No counterpart in the reference: number of register qubits per thread in
  * full tile passes (c128: 3 or 4, c64: 4 or 5; 0 keeps every gate on the LDS
  * tile). */
@@ -213,6 +228,24 @@ int aerhip_qv_inner_product(aerhip_qv *a, aerhip_qv *b, double *re, double *im);
 
 int aerhip_norm(aerhip_qv *qv, double *out);
 
+/* QubitVector::norm(qubits, mat) (qubitvector.hpp:1889-2054): |M psi|^2 for
+ * the 2^nq x 2^nq column-major matrix `mat` on `qubits`, without applying it
+ * (State::apply_kraus, statevector_state.hpp:1136).  nq <= 10. */
+int aerhip_norm_matrix(aerhip_qv *qv, const uint64_t *qubits, uint64_t nq, const double *mat,
+                       double *out);
+
+/* QubitVector::norm_diagonal(qubits, diag) (qubitvector.hpp:1946-2082):
+ * |D psi|^2 for the diagonal `diag` (2^nq entries) on `qubits`.  nq <= 10. */
+int aerhip_norm_diagonal(aerhip_qv *qv, const uint64_t *qubits, uint64_t nq, const double *diag,
+                         double *out);
+
+/* QubitVector::get_state(pos) (qubitvector.hpp:838-841): amplitude `pos` as
+ * (re, im) in out2[0..1]. */
+int aerhip_qv_get_state(aerhip_qv *qv, uint64_t pos, double *out2);
+
+/* QubitVector::set_state(pos, val) (qubitvector.hpp:833-836). */
+int aerhip_qv_set_state(aerhip_qv *qv, uint64_t pos, double re, double im);
+
 /* QubitVector::probability(outcome) (qubitvector.hpp:2090-2092). */
 int aerhip_probability(aerhip_qv *qv, uint64_t outcome, double *out);
 
@@ -287,9 +320,9 @@ typedef struct aerhip_run_config {
   int32_t device;
   uint64_t shots;
   uint64_t seed;            /* seed_simulator of this circuit */
-  int32_t fusion_enable;    /* 1: run the engine's fusion before execution */
-  int32_t fusion_max_qubit; /* Aer default 5 (fusion.hpp:761) */
-  int32_t fusion_threshold; /* Aer default 14 (fusion.hpp:762) */
+  int32_t fusion_enable;    /* 1: the engine fuses diagonal runs (aerhip_qv_set_fusion) */
+  int32_t fusion_max_qubit; /* qubits of one fused op; Aer default 5 (fusion.hpp:761) */
+  int32_t fusion_threshold; /* fuse above this many qubits; Aer default 14 (fusion.hpp:762) */
   int32_t sample_index_size;/* Aer default 10 (config.hpp:108) */
   double global_phase;      /* circuit global phase, applied at
                                initialisation (statevector_state.hpp:446-450) */

@@ -7,17 +7,32 @@
 // Controller::make_circuit_executor, src/controllers/aer_controller.hpp:
 // 624-649, and AerState::initialize_qreg_state,
 // src/controllers/state_controller.hpp:666-677).  Method names, argument
-// meaning and error behaviour (std::runtime_error / std::invalid_argument
-// with the engine's message) follow QubitVector
-// (src/simulators/statevector/qubitvector.hpp:61-651).
+// types, return types and error behaviour (std::runtime_error /
+// std::invalid_argument with the engine's message) follow QubitVector
+// (src/simulators/statevector/qubitvector.hpp:61-470).
+//
+// The header does not include the reference's headers.  Two template
+// parameters connect it to them:
+//   - vector_t: the container copy_to_vector / move_to_vector return; the
+//     reference returns AER::Vector<std::complex<data_t>>
+//     (qubitvector.hpp:124-127), which has the (size_t, bool) constructor and
+//     data() used here;
+//   - the rotation axis is taken as any enum whose enumerators are in the
+//     order of QV::Rotation {x, y, z, xx, yy, zz, zx} (qubitvector.hpp:47-55),
+//     the order of AERHIP_ROT_*.
+// Operations::Op, RngEngine and the classical-register types of the batched
+// and conditional hooks are template parameters for the same reason.
 //
 // Header only; link against libaerhip.so.
 #pragma once
 
 #include <complex>
 #include <cstdint>
+#include <cstdio>
+#include <map>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -25,71 +40,163 @@
 
 namespace aerhip {
 
-template <typename data_t = double>
+template <typename data_t = double,
+          typename vector_t = std::vector<std::complex<data_t>>>
 class QubitVectorHip {
  public:
-  using reg_t = std::vector<uint64_t>;
-  using cvector_t = std::vector<std::complex<double>>;
+  using uint_t = uint64_t;
+  using int_t = int64_t;
+  using reg_t = std::vector<uint_t>;
+  template <typename T>
+  using cvector_t = std::vector<std::complex<T>>;
+  using cdict_t = std::map<std::string, std::complex<data_t>>;
+  using complex_t = std::complex<double>;
 
   QubitVectorHip() = default;
   explicit QubitVectorHip(size_t num_qubits) { set_num_qubits(num_qubits); }
-  ~QubitVectorHip() { release(); }
-  QubitVectorHip(const QubitVectorHip &) = delete;
+  virtual ~QubitVectorHip() { release(); }
+  // As the reference's copy constructor (qubitvector.hpp:73): an empty vector.
+  QubitVectorHip(const QubitVectorHip &) {}
   QubitVectorHip &operator=(const QubitVectorHip &) = delete;
-  QubitVectorHip(QubitVectorHip &&o) noexcept : h_(std::exchange(o.h_, nullptr)), n_(o.n_) {}
+  QubitVectorHip(QubitVectorHip &&o) noexcept { take(o); }
   QubitVectorHip &operator=(QubitVectorHip &&o) noexcept {
-    release();
-    h_ = std::exchange(o.h_, nullptr);
-    n_ = o.n_;
+    if (this != &o) {
+      release();
+      take(o);
+    }
     return *this;
   }
 
   static constexpr int precision() {
     return sizeof(data_t) == sizeof(double) ? AERHIP_PRECISION_DOUBLE : AERHIP_PRECISION_SINGLE;
   }
-  std::string name() const { return "statevector_hip"; }
+  static std::string name() { return "statevector"; }
 
   // -- lifecycle (qubitvector.hpp:861-1109) ---------------------------------
-  void set_num_qubits(size_t n) {
+  virtual void set_num_qubits(size_t n) {
     release();
     h_ = aerhip_qv_create(n, precision(), device_);
     if (!h_) throw std::runtime_error(aerhip_last_error());
     n_ = n;
+    if (index_size_ > 0) ck(aerhip_qv_set_sample_measure_index_size(h_, index_size_));
   }
-  size_t num_qubits() const { return n_; }
-  size_t size() const { return size_t(1) << n_; }
-  size_t required_memory_mb(uint64_t n) const { return aerhip_required_memory_mb(n, precision()); }
-  void set_target_gpus(const reg_t &t) { device_ = t.empty() ? 0 : static_cast<int>(t[0]); }
+  virtual uint_t num_qubits() const { return n_; }
+  uint_t size() const { return uint_t(1) << n_; }
+  size_t required_memory_mb(uint_t n) const { return aerhip_required_memory_mb(n, precision()); }
+  bool top_of_group() { return true; }
   void initialize() { ck(aerhip_qv_initialize(h_)); }
+  // QubitVector::initialize(const QubitVector&) (qubitvector.hpp:199): copy
+  // of another handle's state (device to device).
+  void initialize(const QubitVectorHip &obj) {
+    if (obj.n_ != n_) set_num_qubits(obj.n_);
+    void *src = nullptr;
+    ck(aerhip_qv_device_pointer(obj.h_, &src));
+    ck(aerhip_qv_initialize_from_device(h_, src, size(), obj.precision()));
+  }
   void zero() { ck(aerhip_qv_zero(h_)); }
   template <typename list_t>
   void initialize_from_vector(const list_t &vec) {
     std::vector<std::complex<double>> v(vec.begin(), vec.end());
     ck(aerhip_qv_initialize_from_vector(h_, reinterpret_cast<const double *>(v.data()), v.size()));
   }
-  void initialize_component(const reg_t &qubits, const cvector_t &state) {
+  template <typename list_t>
+  void move_from_vector(list_t &&vec) {
+    initialize_from_vector(vec);
+  }
+  void initialize_from_data(const std::complex<data_t> *data, const size_t num_states) {
+    std::vector<std::complex<double>> v(data, data + num_states);
+    ck(aerhip_qv_initialize_from_vector(h_, reinterpret_cast<const double *>(v.data()), v.size()));
+  }
+  void initialize_component(const reg_t &qubits, const cvector_t<double> &state) {
     ck(aerhip_qv_initialize_component(h_, qubits.data(), qubits.size(),
                                       reinterpret_cast<const double *>(state.data())));
   }
-  std::vector<std::complex<double>> copy_to_vector() const {
+  // The reference hands its host buffer over (qubitvector.hpp:866-873); the
+  // device state is copied out and the handle keeps its memory.
+  vector_t copy_to_vector() const {
     std::vector<std::complex<double>> v(size());
     ck(aerhip_qv_copy_to_vector(h_, reinterpret_cast<double *>(v.data()), v.size()));
-    return v;
+    vector_t out(v.size(), false);
+    for (size_t i = 0; i < v.size(); ++i) out.data()[i] = std::complex<data_t>(v[i]);
+    return out;
+  }
+  vector_t move_to_vector() { return copy_to_vector(); }
+  cvector_t<data_t> vector() const {
+    std::vector<std::complex<double>> v(size());
+    ck(aerhip_qv_copy_to_vector(h_, reinterpret_cast<double *>(v.data()), v.size()));
+    return cvector_t<data_t>(v.begin(), v.end());
+  }
+  // QubitVector::vector_ket (qubitvector.hpp:856-858 -> Utils::vec2ket with
+  // base 16): hex keys of the amplitudes whose chopped value exceeds epsilon.
+  cdict_t vector_ket(double epsilon = 0) const {
+    const cvector_t<data_t> v = vector();
+    cdict_t ket;
+    for (size_t k = 0; k < v.size(); ++k) {
+      auto chop = [epsilon](data_t x) { return std::abs(x) < epsilon ? data_t(0) : x; };
+      const std::complex<data_t> val(chop(v[k].real()), chop(v[k].imag()));
+      if (std::abs(val) > epsilon) {
+        char key[24];
+        std::snprintf(key, sizeof(key), "0x%llx", static_cast<unsigned long long>(k));
+        ket.insert({key, val});
+      }
+    }
+    return ket;
+  }
+  std::complex<data_t> get_state(uint_t pos) const {
+    double v[2];
+    ck(aerhip_qv_get_state(h_, pos, v));
+    return std::complex<data_t>(static_cast<data_t>(v[0]), static_cast<data_t>(v[1]));
+  }
+  void set_state(uint_t pos, std::complex<data_t> &val) {
+    ck(aerhip_qv_set_state(h_, pos, val.real(), val.imag()));
   }
   void synchronize() const { ck(aerhip_qv_synchronize(h_)); }
-  void set_sample_measure_index_size(int n) { ck(aerhip_qv_set_sample_measure_index_size(h_, n)); }
+
+  // -- configuration (qubitvector.hpp:147-170, 421-458) ----------------------
+  void set_sample_measure_index_size(int n) {
+    index_size_ = n;
+    if (h_) ck(aerhip_qv_set_sample_measure_index_size(h_, n));
+  }
+  int get_sample_measure_index_size() { return index_size_; }
   // OpenMP settings of the CPU vector have no meaning on the device.
-  void set_omp_threads(int) {}
-  void set_omp_threshold(int) {}
-  void set_json_chop_threshold(double) {}
+  void set_omp_threads(int n) { omp_threads_ = n; }
+  uint_t get_omp_threads() { return omp_threads_; }
+  void set_omp_threshold(int n) { omp_threshold_ = n; }
+  uint_t get_omp_threshold() { return omp_threshold_; }
+  void set_num_threads_per_group(int) {}
+  void set_json_chop_threshold(double t) { chop_threshold_ = t; }
+  double get_json_chop_threshold() { return chop_threshold_; }
+  // The engine plans matrix sizes and sampling itself.
+  void set_max_matrix_bits(int_t) {}
+  void set_max_sampling_shots(int_t) {}
+  void cuStateVec_enable(bool) {}
+  template <typename reg_list_t>
+  void set_target_gpus(reg_list_t &t) {
+    device_ = t.empty() ? 0 : static_cast<int>(t[0]);
+  }
+  // One chunk holds the whole state (no chunk exchange in this engine).
+  uint_t chunk_setup(int, int, uint_t chunk_index, uint_t) {
+    chunk_index_ = chunk_index;
+    return 1;
+  }
+  uint_t chunk_setup(QubitVectorHip &, const uint_t chunk_index) {
+    chunk_index_ = chunk_index;
+    return 1;
+  }
+  uint_t chunk_index(void) { return chunk_index_; }
+  bool fetch_chunk(void) const { return true; }
+  void release_chunk(bool = true) const {}
+  // The engine blocks gates into tile passes on its own.
+  void enter_register_blocking(const reg_t &) {}
+  void leave_register_blocking(void) {}
   bool enable_batch(bool) const { return false; }
-  bool support_global_indexing() const { return false; }
+  bool support_global_indexing(void) { return false; }
 
   // -- gates (qubitvector.hpp:1299-1871, 2393-2433) --------------------------
-  void apply_matrix(const reg_t &q, const cvector_t &mat) {
+  void apply_matrix(const reg_t &q, const cvector_t<double> &mat) {
     ck(aerhip_apply_matrix(h_, q.data(), q.size(), reinterpret_cast<const double *>(mat.data())));
   }
-  void apply_diagonal_matrix(const reg_t &q, const cvector_t &diag) {
+  void apply_diagonal_matrix(const reg_t &q, const cvector_t<double> &diag) {
     ck(aerhip_apply_diagonal_matrix(h_, q.data(), q.size(),
                                     reinterpret_cast<const double *>(diag.data())));
   }
@@ -99,31 +206,41 @@ class QubitVectorHip {
   void apply_mcphase(const reg_t &q, const std::complex<double> phase) {
     ck(aerhip_apply_mcphase(h_, q.data(), q.size(), phase.real(), phase.imag()));
   }
-  void apply_mcu(const reg_t &q, const cvector_t &mat) {
+  void apply_mcu(const reg_t &q, const cvector_t<double> &mat) {
     ck(aerhip_apply_mcu(h_, q.data(), q.size(), reinterpret_cast<const double *>(mat.data())));
   }
-  // axis: AERHIP_ROT_* in the order of QV::Rotation (qubitvector.hpp:47-55)
-  void apply_rotation(const reg_t &q, int axis, double theta) {
-    ck(aerhip_apply_rotation(h_, q.data(), q.size(), axis, theta));
+  // r: QV::Rotation (or any enum / integer in its order).
+  template <typename rotation_t>
+  void apply_rotation(const reg_t &q, const rotation_t r, const double theta) {
+    static_assert(std::is_enum<rotation_t>::value || std::is_integral<rotation_t>::value,
+                  "the rotation axis is QV::Rotation");
+    ck(aerhip_apply_rotation(h_, q.data(), q.size(), static_cast<int>(r), theta));
   }
-  void apply_pauli(const reg_t &q, const std::string &pauli,
-                   const std::complex<double> &coeff = 1) {
+  void apply_pauli(const reg_t &q, const std::string &pauli, const complex_t &coeff = 1) {
     ck(aerhip_apply_pauli(h_, q.data(), q.size(), pauli.c_str(), coeff.real(), coeff.imag()));
   }
+  template <typename pair_uint_t>
   void apply_permutation_matrix(const reg_t &q,
-                                const std::vector<std::pair<uint64_t, uint64_t>> &pairs) {
+                                const std::vector<std::pair<pair_uint_t, pair_uint_t>> &pairs) {
     std::vector<uint64_t> flat;
+    flat.reserve(2 * pairs.size());
     for (const auto &p : pairs) {
       flat.push_back(p.first);
       flat.push_back(p.second);
     }
     ck(aerhip_apply_permutation_matrix(h_, q.data(), q.size(), flat.data(), pairs.size()));
   }
-  void apply_multiplexer(const reg_t &controls, const reg_t &targets, const cvector_t &mat) {
+  void apply_multiplexer(const reg_t &controls, const reg_t &targets, const cvector_t<double> &mat) {
     ck(aerhip_apply_multiplexer(h_, controls.data(), controls.size(), targets.data(),
                                 targets.size(), reinterpret_cast<const double *>(mat.data())));
   }
   void apply_multi_swaps(const reg_t &q) { ck(aerhip_apply_multi_swaps(h_, q.data(), q.size())); }
+  // Chunk exchange between devices is outside this engine (one device holds
+  // the whole state).
+  template <typename... Args>
+  void apply_chunk_swap(Args &&...) {
+    throw std::runtime_error("QubitVectorHip: chunk exchange is not supported");
+  }
 
   // -- readout (qubitvector.hpp:1879-2347) ---------------------------------
   double norm() const {
@@ -131,7 +248,25 @@ class QubitVectorHip {
     ck(aerhip_norm(h_, &v));
     return v;
   }
-  double probability(uint64_t outcome) const {
+  double norm(const uint_t qubit, const cvector_t<double> &mat) const {
+    return norm(reg_t({qubit}), mat);
+  }
+  double norm(const reg_t &qubits, const cvector_t<double> &mat) const {
+    double v;
+    ck(aerhip_norm_matrix(h_, qubits.data(), qubits.size(),
+                          reinterpret_cast<const double *>(mat.data()), &v));
+    return v;
+  }
+  double norm_diagonal(const uint_t qubit, const cvector_t<double> &mat) const {
+    return norm_diagonal(reg_t({qubit}), mat);
+  }
+  double norm_diagonal(const reg_t &qubits, const cvector_t<double> &mat) const {
+    double v;
+    ck(aerhip_norm_diagonal(h_, qubits.data(), qubits.size(),
+                            reinterpret_cast<const double *>(mat.data()), &v));
+    return v;
+  }
+  double probability(const uint_t outcome) const {
     double v;
     ck(aerhip_probability(h_, outcome, &v));
     return v;
@@ -152,25 +287,92 @@ class QubitVectorHip {
     return out;
   }
   double expval_pauli(const reg_t &q, const std::string &pauli,
-                      const std::complex<double> initial_phase = 1.0) const {
+                      const complex_t initial_phase = 1.0) const {
     double v;
     ck(aerhip_expval_pauli(h_, q.data(), q.size(), pauli.c_str(), initial_phase.real(),
                            initial_phase.imag(), &v));
     return v;
   }
 
+  // -- batched / conditional hooks of the GPU executor -----------------------
+  // (qubitvector.hpp:322-360, 413-416).  The engine runs one state per
+  // handle, so batched execution is reported unsupported and the executors
+  // take their per-state paths; the batched entry points are never reached
+  // and throw if they are.
+  virtual bool batched_optimization_supported(void) { return false; }
+  template <typename op_t>
+  void apply_bfunc(const op_t &) {
+    unsupported("apply_bfunc");
+  }
+  void set_conditional(int_t) {}
+  template <typename op_t, typename rng_list_t>
+  void apply_roerror(const op_t &, rng_list_t &) {
+    unsupported("apply_roerror");
+  }
+  template <typename rng_list_t>
+  void apply_batched_measure(const reg_t &, rng_list_t &, const reg_t &, const reg_t &) {
+    unsupported("apply_batched_measure");
+  }
+  template <typename rng_list_t>
+  void apply_batched_reset(const reg_t &, rng_list_t &) {
+    unsupported("apply_batched_reset");
+  }
+  template <typename storage_t>
+  void read_measured_data(storage_t &) {}
+  int_t set_batched_system_conditional(int_t, reg_t &) { return -1; }
+  template <typename op_list_t>
+  void apply_batched_pauli_ops(const op_list_t &) {
+    unsupported("apply_batched_pauli_ops");
+  }
+  template <typename mat_list_t, typename rng_list_t>
+  void apply_batched_kraus(const reg_t &, const mat_list_t &, rng_list_t &) {
+    unsupported("apply_batched_kraus");
+  }
+  void apply_batched_matrix(const reg_t &, const cvector_t<double> &, const uint_t, const uint_t) {
+    unsupported("apply_batched_matrix");
+  }
+  void apply_batched_diagonal_matrix(const reg_t &, const cvector_t<double> &, const uint_t,
+                                     const uint_t) {
+    unsupported("apply_batched_diagonal_matrix");
+  }
+  void batched_expval_pauli(std::vector<double> &, const reg_t &, const std::string &, bool,
+                            std::complex<double>, bool, const complex_t = 1.0) const {
+    unsupported("batched_expval_pauli");
+  }
+  void initialize_creg(uint_t, uint_t) {}
+  void initialize_creg(uint_t, uint_t, const std::string &, const std::string &) {}
+
  private:
   static void ck(int rc) {
     if (rc == 2) throw std::invalid_argument(aerhip_last_error());
     if (rc != 0) throw std::runtime_error(aerhip_last_error());
   }
+  [[noreturn]] static void unsupported(const char *what) {
+    throw std::runtime_error(std::string("QubitVectorHip: ") + what +
+                             " is not supported (batched execution is off)");
+  }
   void release() {
     if (h_) aerhip_qv_destroy(h_);
     h_ = nullptr;
   }
+  void take(QubitVectorHip &o) {
+    h_ = std::exchange(o.h_, nullptr);
+    n_ = o.n_;
+    device_ = o.device_;
+    index_size_ = o.index_size_;
+    chunk_index_ = o.chunk_index_;
+    omp_threads_ = o.omp_threads_;
+    omp_threshold_ = o.omp_threshold_;
+    chop_threshold_ = o.chop_threshold_;
+  }
   aerhip_qv *h_ = nullptr;
   size_t n_ = 0;
   int device_ = 0;
+  int index_size_ = 0;
+  uint_t chunk_index_ = 0;
+  uint_t omp_threads_ = 1;
+  uint_t omp_threshold_ = 14;
+  double chop_threshold_ = 1e-10;
 };
 
 }  // namespace aerhip

@@ -896,4 +896,11 @@ int oracle_fuse_count(const aerhip_op *ops, uint64_t num_ops, uint64_t nq, int m
 
 int oracle_max_threads(void) { return omp_get_max_threads(); }
 
+// Installs (or, with nulls, removes) the reference's compiled transformer for
+// complex128 apply_matrix / apply_diagonal_matrix (the CPU baseline).
+void oracle_set_transformer(void *matrix_fn, void *diagonal_fn) {
+  g_matrix_hook = reinterpret_cast<TransformerHook>(matrix_fn);
+  g_diagonal_hook = reinterpret_cast<TransformerHook>(diagonal_fn);
+}
+
 }  // extern "C"

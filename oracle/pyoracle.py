@@ -82,6 +82,7 @@ def ref():
         L.ref_vmatrix.argtypes = [C.c_char_p, C.c_void_p, C.c_void_p]
         L.ref_rng_uniform.argtypes = [C.c_uint64, C.c_uint64, C.c_void_p]
         L.ref_rng_discrete.argtypes = [C.c_uint64, C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p]
+        L.ref_avx2_supported.restype = C.c_int
         _ref = L
     return _ref
 
@@ -227,6 +228,28 @@ class OracleQubitVector:
         _check(self._L.oracle_expval_pauli(self._h, qp, n, pauli.encode(), phase.real, phase.imag,
                                            C.byref(out)))
         return out.value
+
+
+class reference_transformer:
+    """Context in which the oracle's complex128 apply_matrix /
+    apply_diagonal_matrix run the reference's own compiled Transformer /
+    TransformerAVX2 (oracle/_ref, chosen by the reference's CPUID check) instead
+    of the restatement.  Used by the timed CPU baseline."""
+
+    def __enter__(self):
+        R = ref()
+        if R is None:
+            raise RuntimeError("oracle/_ref is not built")
+        L = lib()
+        L.oracle_set_transformer.argtypes = [C.c_void_p, C.c_void_p]
+        L.oracle_set_transformer(C.cast(R.ref_qv_apply_matrix, C.c_void_p),
+                                 C.cast(R.ref_qv_apply_diagonal_matrix, C.c_void_p))
+        self.avx2 = bool(R.ref_avx2_supported())
+        return self
+
+    def __exit__(self, *exc):
+        lib().oracle_set_transformer(None, None)
+        return False
 
 
 def run_circuit(circ, shots=1024, seed=0, precision="double", fusion=True, threads=0,

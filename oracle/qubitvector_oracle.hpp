@@ -27,6 +27,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 namespace oracle {
@@ -62,6 +63,15 @@ inline void indexes(const reg_t &qs, const reg_t &qs_sorted, uint_t k, uint_t *r
     for (uint_t j = 0; j < n; ++j) ret[n + j] = ret[j] | b;
   }
 }
+
+// Optional transformer of complex128 vectors: the reference's own compiled
+// Transformer / TransformerAVX2 (oracle/_ref), installed by the CPU baseline
+// so that the timed gate kernels are the reference's.  Arguments: data,
+// number of qubits of the state, qubits, number of qubits, column-major
+// matrix (or diagonal) as (re, im) pairs, threads.  Null: the restatement.
+using TransformerHook = int (*)(double *, uint64_t, const uint64_t *, uint64_t, const double *, int);
+inline TransformerHook g_matrix_hook = nullptr;
+inline TransformerHook g_diagonal_hook = nullptr;
 
 template <typename data_t>
 class QubitVector {
@@ -154,6 +164,12 @@ class QubitVector {
   // Transformer::apply_matrix (transformer.hpp:89-164)
   void apply_matrix(const reg_t &qs, const cvector_t<double> &mat) {
     if (qs.size() > 20) throw std::runtime_error("Maximum size of apply matrix is a 20-qubit matrix.");
+    if (std::is_same<data_t, double>::value && g_matrix_hook) {
+      if (g_matrix_hook(reinterpret_cast<double *>(data_), num_qubits_, qs.data(), qs.size(),
+                        reinterpret_cast<const double *>(mat.data()), threads()) != 0)
+        throw std::runtime_error("reference transformer failed");
+      return;
+    }
     if (qs.size() == 1) return apply_matrix_1(qs[0], mat);
     const auto m = convert(mat);
     const size_t D = BIT(qs.size());
@@ -208,6 +224,12 @@ class QubitVector {
 
   // Transformer::apply_diagonal_matrix (transformer.hpp:234-258)
   void apply_diagonal_matrix(const reg_t &qs, const cvector_t<double> &diag) {
+    if (std::is_same<data_t, double>::value && g_diagonal_hook) {
+      if (g_diagonal_hook(reinterpret_cast<double *>(data_), num_qubits_, qs.data(), qs.size(),
+                          reinterpret_cast<const double *>(diag.data()), threads()) != 0)
+        throw std::runtime_error("reference transformer failed");
+      return;
+    }
     if (qs.size() == 1) return apply_diagonal_matrix_1(qs[0], diag);
     const auto d = convert(diag);
     const size_t N = qs.size();

@@ -11,6 +11,7 @@
 #include <cstring>
 #include <string>
 
+#include "framework/avx2_detect.hpp"
 #include "framework/linalg/matrix_utils/vmatrix_defs.hpp"
 #include "framework/rng.hpp"
 #include "simulators/statevector/indexes.hpp"
@@ -56,6 +57,53 @@ int ref_apply_diagonal_matrix(double *data, uint64_t nq_state, const uint64_t *q
     } else {
       AER::QV::Transformer<cd *, double> t;
       t.apply_diagonal_matrix(d, 1ull << nq_state, 1, qs, m);
+    }
+    return 0;
+  } catch (...) {
+    return 1;
+  }
+}
+
+// QubitVector's transformer as the reference selects it at run time
+// (set_transformer_method, qubitvector.hpp:491-502: TransformerAVX2 when
+// is_avx2_supported(), avx2_detect.hpp:54-88, else Transformer), called with
+// the thread count of omp_threads_managed (qubitvector.hpp:486-489).  The
+// CPU baseline times the reference's own gate kernels through these.
+int ref_avx2_supported(void) { return AER::is_avx2_supported() ? 1 : 0; }
+
+int ref_qv_apply_matrix(double *data, uint64_t nq_state, const uint64_t *q, uint64_t nq,
+                        const double *mat, int threads) {
+  try {
+    AER::reg_t qs(q, q + nq);
+    AER::QV::cvector_t<double> m(1ull << (2 * nq));
+    std::memcpy(m.data(), mat, m.size() * sizeof(cd));
+    auto *d = reinterpret_cast<cd *>(data);
+    if (AER::is_avx2_supported()) {
+      AER::QV::TransformerAVX2<cd *, double> t;
+      t.apply_matrix(d, 1ull << nq_state, threads, qs, m);
+    } else {
+      AER::QV::Transformer<cd *, double> t;
+      t.apply_matrix(d, 1ull << nq_state, threads, qs, m);
+    }
+    return 0;
+  } catch (...) {
+    return 1;
+  }
+}
+
+int ref_qv_apply_diagonal_matrix(double *data, uint64_t nq_state, const uint64_t *q, uint64_t nq,
+                                 const double *diag, int threads) {
+  try {
+    AER::reg_t qs(q, q + nq);
+    AER::QV::cvector_t<double> m(1ull << nq);
+    std::memcpy(m.data(), diag, m.size() * sizeof(cd));
+    auto *d = reinterpret_cast<cd *>(data);
+    if (AER::is_avx2_supported()) {
+      AER::QV::TransformerAVX2<cd *, double> t;
+      t.apply_diagonal_matrix(d, 1ull << nq_state, threads, qs, m);
+    } else {
+      AER::QV::Transformer<cd *, double> t;
+      t.apply_diagonal_matrix(d, 1ull << nq_state, threads, qs, m);
     }
     return 0;
   } catch (...) {

@@ -106,6 +106,12 @@ _SIGNATURES = {
     "aerhip_apply_multiplexer": (C.c_int, [C.c_void_p, _u64p, C.c_uint64, _u64p, C.c_uint64, C.c_void_p]),
     "aerhip_apply_multi_swaps": (C.c_int, [C.c_void_p, _u64p, C.c_uint64]),
     "aerhip_norm": (C.c_int, [C.c_void_p, _dp]),
+    "aerhip_qv_set_fusion": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "aerhip_qv_fused_count": (C.c_int, [C.c_void_p, _u64p]),
+    "aerhip_norm_matrix": (C.c_int, [C.c_void_p, _u64p, C.c_uint64, _dp, _dp]),
+    "aerhip_norm_diagonal": (C.c_int, [C.c_void_p, _u64p, C.c_uint64, _dp, _dp]),
+    "aerhip_qv_get_state": (C.c_int, [C.c_void_p, C.c_uint64, _dp]),
+    "aerhip_qv_set_state": (C.c_int, [C.c_void_p, C.c_uint64, C.c_double, C.c_double]),
     "aerhip_qv_inner_product": (C.c_int, [C.c_void_p, C.c_void_p, _dp, _dp]),
     "aerhip_probability": (C.c_int, [C.c_void_p, C.c_uint64, _dp]),
     "aerhip_probabilities": (C.c_int, [C.c_void_p, _u64p, C.c_uint64, C.c_void_p]),

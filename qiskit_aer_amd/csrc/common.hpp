@@ -88,6 +88,8 @@ enum TileOpKind : uint32_t {
   K_PAULI = 7,    // Pauli string: flip xl, sign by parity(z), times phase
   K_PERM = 8,     // swap pairs of basis states of k local qubits
   K_PROBE = 9,    // read-only: tile share of Re<psi|phase P|psi> into probe slot npairs
+  K_PHASES = 10,  // fused diagonal of register passes: where the controls hold, the product
+                  // over npairs members of (bit ? v : u); pool[off + 3m .. +2] = (qubit, u, v)
 };
 
 constexpr uint8_t kNotLocal = 0xFF;
@@ -105,7 +107,7 @@ struct TileOp {
   uint32_t off;  // offset into the coefficient pool (complex<double> units)
   uint64_t zg;   // Pauli: Z mask over global bits
   uint32_t npairs;
-  uint32_t pad0;
+  uint32_t nonunitary;  // 1: the op rescales amplitudes (projector, |phase| != 1)
   double pr, pi;  // phase
   uint8_t pos[16];   // local bit of entry j (kNotLocal when fixed by the tile)
   uint8_t gpos[16];  // global qubit of entry j

@@ -175,20 +175,26 @@ struct Runner {
     const uint64_t meas = std::discrete_distribution<size_t>(probs.begin(), probs.end())(rng);
     const double p = probs[meas];
     const uint64_t final_state = reset ? 0 : meas;
-    G::cvec mdiag(dim, 0.);
-    mdiag[meas] = 1. / std::sqrt(p);
-    diag(q, mdiag);
+    // The projector diag(..., 1/sqrt(p) at meas, ...).  Up to 10 qubits it is
+    // one diagonal, as in the reference; beyond, it is the product of
+    // diagonals on groups of 10 qubits (the first carries 1/sqrt(p), the
+    // others 0/1 entries), which gives every amplitude the same value.
+    const double scale = 1. / std::sqrt(p);
+    for (size_t g0 = 0; g0 < q.size(); g0 += 10) {
+      const size_t gk = std::min<size_t>(10, q.size() - g0);
+      const std::vector<uint64_t> gq(q.begin() + g0, q.begin() + g0 + gk);
+      G::cvec mdiag(1ull << gk, 0.);
+      mdiag[(meas >> g0) & ((1ull << gk) - 1)] = g0 == 0 ? cd(scale) : cd(1.);
+      diag(gq, mdiag);
+    }
     if (final_state != meas) {
-      if (q.size() == 1) {
-        ck(aerhip_apply_mcx(qv, q.data(), q.size()));
-      } else {
-        G::cvec perm(dim * dim, 0.);
-        perm[final_state * dim + meas] = 1.;
-        perm[meas * dim + final_state] = 1.;
-        for (uint64_t j = 0; j < dim; ++j)
-          if (j != final_state && j != meas) perm[j * dim + j] = 1.;
-        matrix(q, perm);
-      }
+      // The reference swaps the basis states meas and final_state of the
+      // measured qubits (an X for one qubit, a permutation matrix for more,
+      // statevector_state.hpp:985-1011).  After the projection every other
+      // basis state of those qubits holds exact zeros, so an X on each qubit
+      // where meas and final_state differ moves the same values.
+      for (size_t i = 0; i < q.size(); ++i)
+        if (((meas ^ final_state) >> i) & 1ull) ck(aerhip_apply_mcx(qv, &q[i], 1));
     }
     return meas;
   }
@@ -342,6 +348,8 @@ int aerhip_run_circuit(const aerhip_op *ops, uint64_t num_ops, uint64_t num_qubi
     R.qv = aerhip_qv_create(num_qubits, cfg.precision, cfg.device);
     if (!R.qv) throw std::runtime_error(aerhip_last_error());
     ck(aerhip_qv_set_sample_measure_index_size(R.qv, cfg.sample_index_size));
+    ck(aerhip_qv_set_fusion(R.qv, cfg.fusion_enable, std::max(2, cfg.fusion_max_qubit),
+                            std::max(0, cfg.fusion_threshold)));
 
     uint64_t first_meas = num_ops;
     for (uint64_t i = 0; i < num_ops; ++i)

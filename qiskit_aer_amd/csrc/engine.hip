@@ -2,6 +2,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 
@@ -398,6 +399,7 @@ void Engine::apply_mcphase(const std::vector<uint32_t> &qubits, cd phase) {
   p.ctrl = qubits;
   p.pr = phase.real();
   p.pi = phase.imag();
+  p.unitary = std::abs(std::abs(phase) - 1.0) <= 1e-12;
   push(std::move(p));
 }
 
@@ -493,13 +495,17 @@ void Engine::apply_pauli(const std::vector<uint32_t> &qubits, const std::string 
   const uint64_t lowmask = (1ull << low_bits_) - 1ull;
   if (static_cast<uint32_t>(__builtin_popcountll(m.x | lowmask)) > tile_bits_) {
     // X/Y on more qubits than a tile holds: the sign-and-phase part as a
-    // Z-only string, then one X per qubit (the values are the same: each
-    // amplitude is multiplied by the same +-phase and moved).
+    // Z-only string, then one X per qubit.  The reference takes the sign from
+    // the index an amplitude moves to (qubitvector.hpp:2421-2430); applied
+    // before the move it is taken from the source index i ^ x, which differs
+    // by parity(x & z) = num_y mod 2, so the phase carries (-1)^num_y.
     Prim d;
     d.kind = K_PAULI;
     d.zmask = m.z;
+    if (m.num_y & 1) ph = -ph;
     d.pr = ph.real();
     d.pi = ph.imag();
+    d.unitary = std::abs(std::abs(ph) - 1.0) <= 1e-12;
     push(std::move(d));
     for (uint32_t q = 0; q < n_; ++q)
       if (m.x & bit(q)) apply_mcx({q});
@@ -512,6 +518,7 @@ void Engine::apply_pauli(const std::vector<uint32_t> &qubits, const std::string 
   p.tmask = m.x;
   p.pr = ph.real();
   p.pi = ph.imag();
+  p.unitary = std::abs(std::abs(ph) - 1.0) <= 1e-12;
   // pivot qubit: the highest X/Y qubit (x_max of the reference)
   p.q = {m.x_max};
   if (m.x == 0) p.q.clear();
@@ -587,15 +594,54 @@ void Engine::apply_multi_swaps(const std::vector<uint32_t> &qubits) {
 void Engine::initialize_component(const std::vector<uint32_t> &qubits,
                                   const std::vector<cd> &state) {
   check_qubits(qubits);
+  if (qubits.empty()) return;
+  if (qubits.size() > 40) throw std::invalid_argument("aerhip: initialize_component size");
   const uint64_t D = 1ull << qubits.size();
   if (state.size() != D) throw std::invalid_argument("aerhip: initialize_component size");
-  std::vector<cd> mat(D * D, 0.0);
-  for (uint64_t i = 0; i < D; ++i) mat[i] = state[i];
-  if (qubits.size() == 1) {
-    push_dense(qubits, {}, ~0ull, mat.data());
+  if (qubits.size() <= 5) {
+    // Up to 5 qubits: queued as the dense matrix whose first column is
+    // `state` (its other entries are zeros, so the products are the same).
+    std::vector<cd> mat(D * D, 0.0);
+    for (uint64_t i = 0; i < D; ++i) mat[i] = state[i];
+    if (qubits.size() == 1)
+      push_dense(qubits, {}, ~0ull, mat.data());
+    else
+      apply_matrix(qubits, mat);
     return;
   }
-  apply_matrix(qubits, mat);
+  flush();
+  AERHIP_CHECK(hipSetDevice(device_));
+  MarginalDesc md;
+  std::memset(&md, 0, sizeof(md));
+  std::vector<uint32_t> sorted = qubits;
+  std::sort(sorted.begin(), sorted.end());
+  md.nq = static_cast<uint32_t>(qubits.size());
+  md.inner = size_ >> qubits.size();
+  for (uint32_t i = 0; i < md.nq; ++i) {
+    md.sorted[i] = static_cast<uint8_t>(sorted[i]);
+    md.order[i] = static_cast<uint8_t>(qubits[i]);
+  }
+  const size_t amp = prec_ == AERHIP_PRECISION_DOUBLE ? 16 : 8;
+  const size_t desc_off = 0, state_off = 256, cache_off = align_up(state_off + D * sizeof(cd), 256);
+  char *sc = static_cast<char *>(scratch(cache_off + md.inner * amp));
+  AERHIP_CHECK(hipMemcpyAsync(sc + desc_off, &md, sizeof(md), hipMemcpyHostToDevice, stream_));
+  AERHIP_CHECK(hipMemcpyAsync(sc + state_off, state.data(), D * sizeof(cd), hipMemcpyHostToDevice,
+                              stream_));
+  const auto *dmd = reinterpret_cast<const MarginalDesc *>(sc + desc_off);
+  const auto *dst = reinterpret_cast<const cx<double> *>(sc + state_off);
+  if (prec_ == AERHIP_PRECISION_DOUBLE) {
+    auto *sv = static_cast<cx<double> *>(sv_);
+    auto *cache = reinterpret_cast<cx<double> *>(sc + cache_off);
+    k_gather_group0<double><<<grid_for(md.inner, 16384), kThreads, 0, stream_>>>(sv, dmd, cache);
+    k_init_component<double><<<grid_for(size_, 16384), kThreads, 0, stream_>>>(sv, dmd, size_, cache, dst);
+  } else {
+    auto *sv = static_cast<cx<float> *>(sv_);
+    auto *cache = reinterpret_cast<cx<float> *>(sc + cache_off);
+    k_gather_group0<float><<<grid_for(md.inner, 16384), kThreads, 0, stream_>>>(sv, dmd, cache);
+    k_init_component<float><<<grid_for(size_, 16384), kThreads, 0, stream_>>>(sv, dmd, size_, cache, dst);
+  }
+  AERHIP_CHECK(hipGetLastError());
+  AERHIP_CHECK(hipStreamSynchronize(stream_));
 }
 
 // ---------------------------------------------------------------------------
@@ -623,6 +669,8 @@ struct RItem {
   uint32_t need;  // tile bits that must be register bits
   uint32_t touch; // tile bits read or written
 };
+
+constexpr uint32_t kProbeSentinel = 1u << 31;
 
 RItem classify_item(const TileOp &op, uint32_t R) {
   RItem it;
@@ -658,6 +706,11 @@ RItem classify_item(const TileOp &op, uint32_t R) {
       it.cls = 1;
       it.cat = 1;
       break;
+    case K_PHASES:
+      it.cls = 1;
+      it.cat = 1;
+      it.touch |= op.xl;  // tile bits of the members
+      break;
     case K_PROBE:
       it.cat = 2;
       it.touch = op.xl | op.zl;
@@ -673,6 +726,11 @@ RItem classify_item(const TileOp &op, uint32_t R) {
       it.touch |= pos;
       break;
   }
+  // A probe never moves across a non-unitary op (a projector or a scale
+  // rescales every expectation value): probes carry the sentinel bit 31 (no
+  // tile bit), non-unitary ops touch every bit.
+  if (op.kind == K_PROBE) it.touch |= kProbeSentinel;
+  if (op.nonunitary) it.touch = ~0u;
   return it;
 }
 
@@ -715,6 +773,155 @@ RLayout make_layout(uint32_t rs, uint32_t B, const uint8_t *spos) {
 }
 
 }  // namespace
+
+// Engine-side diagonal fusion (the counterpart of Aer's DiagonalFusion,
+// src/transpile/fusion.hpp:520-726, for this engine's tile passes).  Within
+// one register pass, phase gates on one or two qubits (mcphase, i.e. p / cp /
+// cz / cu1 / z / s / t) and uncontrolled single-qubit diagonals (rz, u1 ...)
+// are gathered into groups whose product is one K_PHASES op:
+//   - a gate on two qubits {a, b} joins the open group whose pivot is a or b
+//     (its other qubit becomes a member), or opens a group pivoted on the
+//     higher qubit (the target of a controlled-phase ladder);
+//   - a single-qubit phase or diagonal joins the one group without a pivot.
+// A member contributes u where its qubit is 0 and v where it is 1; repeated
+// members multiply.  An op that is not fused closes (emits) every open group
+// that shares a qubit with it, just before it: members are moved later only
+// past ops on other qubits, which commute with them.  Probes close every
+// group.  A group that holds a single op is emitted as that op.  `max_qubits`
+// bounds the qubits of one group (pivot and members).  Member lists go to
+// `pool` as (qubit, u, v) triples.
+std::vector<TileOp> fuse_phase_runs(const std::vector<TileOp> &in, const uint8_t *spos, uint32_t ns,
+                                    std::vector<cd> &pool, uint32_t max_qubits,
+                                    uint64_t *fused_ops) {
+  struct Group {
+    int pivot = -1;
+    uint64_t qmask = 0;
+    std::vector<uint32_t> order;  // member qubits in first-seen order
+    std::map<uint32_t, std::pair<cd, cd>> members;
+    std::vector<TileOp> originals;
+    bool nonunitary = false;
+  };
+  int loc[64];
+  for (int &v : loc) v = -1;
+  for (uint32_t b = 0; b < ns; ++b) loc[spos[b]] = static_cast<int>(b);
+  auto local_qubits = [&](uint32_t lmask) {
+    uint64_t g = 0;
+    for (uint32_t b = 0; b < ns; ++b)
+      if (lmask & (1u << b)) g |= 1ull << spos[b];
+    return g;
+  };
+  std::vector<Group> open;
+  std::vector<TileOp> out;
+  out.reserve(in.size());
+  auto emit = [&](Group &g) {
+    if (g.originals.size() == 1) {
+      out.push_back(g.originals[0]);
+      return;
+    }
+    TileOp op;
+    std::memset(&op, 0, sizeof(op));
+    op.kind = K_PHASES;
+    if (g.pivot >= 0) {
+      const int lb = loc[g.pivot];
+      if (lb >= 0) {
+        op.cl = op.cvl = 1u << lb;
+      } else {
+        op.cg = op.cvg = 1ull << g.pivot;
+      }
+    }
+    op.off = static_cast<uint32_t>(pool.size());
+    op.npairs = static_cast<uint32_t>(g.order.size());
+    for (uint32_t q : g.order) {
+      const auto &uv = g.members[q];
+      pool.push_back(cd(static_cast<double>(q), 0.0));
+      pool.push_back(uv.first);
+      pool.push_back(uv.second);
+      if (loc[q] >= 0) op.xl |= 1u << loc[q];
+    }
+    op.nonunitary = g.nonunitary ? 1u : 0u;
+    out.push_back(op);
+    if (fused_ops) *fused_ops += g.originals.size();
+  };
+  auto close_if = [&](uint64_t touch) {
+    std::vector<Group> keep;
+    for (Group &g : open) {
+      if (g.qmask & touch)
+        emit(g);
+      else
+        keep.push_back(std::move(g));
+    }
+    open.swap(keep);
+  };
+  auto add = [&](int pivot, uint32_t q, cd u, cd v, const TileOp &orig) {
+    const uint64_t need = (pivot >= 0 ? 1ull << pivot : 0ull) | (1ull << q);
+    Group *g = nullptr;
+    for (Group &c : open)
+      if (c.pivot == pivot) g = &c;
+    if (g && static_cast<uint32_t>(__builtin_popcountll(g->qmask | need)) > max_qubits) {
+      close_if(g->qmask);
+      g = nullptr;
+    }
+    if (!g) {
+      open.emplace_back();
+      g = &open.back();
+      g->pivot = pivot;
+    }
+    g->qmask |= need;
+    auto it = g->members.find(q);
+    if (it == g->members.end()) {
+      g->members[q] = {u, v};
+      g->order.push_back(q);
+    } else {
+      it->second.first *= u;
+      it->second.second *= v;
+    }
+    g->originals.push_back(orig);
+    g->nonunitary = g->nonunitary || orig.nonunitary;
+  };
+  for (const TileOp &op : in) {
+    if (op.kind == K_MCPHASE && op.cvl == op.cl && op.cvg == op.cg) {
+      const uint64_t qs = local_qubits(op.cl) | op.cg;
+      const int nq = __builtin_popcountll(qs);
+      const cd ph(op.pr, op.pi);
+      if (nq == 1) {
+        add(-1, static_cast<uint32_t>(__builtin_ctzll(qs)), cd(1.0, 0.0), ph, op);
+        continue;
+      }
+      if (nq == 2) {
+        const uint32_t a = static_cast<uint32_t>(__builtin_ctzll(qs));
+        const uint32_t b = 63u - static_cast<uint32_t>(__builtin_clzll(qs));
+        bool has_a = false, has_b = false;
+        for (const Group &g : open) {
+          has_a = has_a || g.pivot == static_cast<int>(a);
+          has_b = has_b || g.pivot == static_cast<int>(b);
+        }
+        if (has_b || !has_a)
+          add(static_cast<int>(b), a, cd(1.0, 0.0), ph, op);
+        else
+          add(static_cast<int>(a), b, cd(1.0, 0.0), ph, op);
+        continue;
+      }
+    }
+    if (op.kind == K_DIAG && op.k == 1 && op.cl == 0 && op.cg == 0) {
+      const uint32_t q = op.pos[0] != kNotLocal ? spos[op.pos[0]] : op.gpos[0];
+      add(-1, q, pool[op.off], pool[op.off + 1], op);
+      continue;
+    }
+    if (op.kind == K_PROBE) {
+      close_if(~0ull);
+      out.push_back(op);
+      continue;
+    }
+    uint64_t touch = local_qubits(op.cl | op.xl | op.zl) | op.cg | op.zg;
+    for (uint32_t j = 0; j < op.k && j < 16; ++j) touch |= 1ull << op.gpos[j];
+    if (op.kind == K_DENSE1 || op.kind == K_MCX || op.kind == K_MCY || op.kind == K_PAULI)
+      touch |= local_qubits(1u << op.t);
+    close_if(touch);
+    out.push_back(op);
+  }
+  close_if(~0ull);
+  return out;
+}
 
 // Walks a register program the way k_reg_pass does and checks its shape and
 // every layout, so that a malformed program fails on the host and never
@@ -779,6 +986,7 @@ void validate_register_program(const ROp<double> *p, size_t n, uint32_t B, uint3
         case Q_DIAG: if (r.extra >= ntops) fail("diagonal index"); break;
         case Q_PROBE: if (r.u.pr.xr >= (1u << R)) fail("probe mask"); break;
         case Q_PHASE: case Q_NOP: break;
+        case Q_DIAGF: if (r.a > 8) fail("fused diagonal tables"); break;
         default: fail("op kind inside a register phase");
       }
     }
@@ -802,7 +1010,7 @@ void validate_register_program(const ROp<double> *p, size_t n, uint32_t B, uint3
 // `tops`.  `lowb`: tile bits that must be lane bits for a full 128-byte run per
 // HBM access.
 void register_program(const std::vector<TileOp> &in, uint32_t B, uint32_t R, uint32_t lowb,
-                      const uint8_t *spos, const std::vector<cd> &pool,
+                      const uint8_t *spos, std::vector<cd> &pool,
                       std::vector<ROp<double>> &rops, std::vector<TileOp> &tops,
                       uint64_t *relayouts) {
   std::vector<RItem> rem;
@@ -1111,6 +1319,58 @@ void register_program(const std::vector<TileOp> &in, uint32_t B, uint32_t R, uin
         r.u.m[0] = op.pr;
         r.u.m[1] = op.pi;
         break;
+      case K_PHASES: {
+        // Tables of the fused diagonal for the layout in force: register
+        // factors R_j, lane factors over thread bits 0-3 and 4-7, and
+        // 16-entry tables over groups of four fixed qubits.
+        materialize(reg_bits(op.cl | op.xl));
+        r.kind = Q_DIAGF;
+        r.b = 0;
+        for (uint32_t j = 0; j < (1u << R); ++j)
+          if ((reg_tile_bits(j) & op.cl & cur_rs) == (op.cvl & op.cl & cur_rs)) r.b |= 1u << j;
+        int loc[64];
+        for (int &v : loc) v = -1;
+        for (uint32_t b = 0; b < B; ++b) loc[spos[b]] = static_cast<int>(b);
+        uint32_t tp[16], nt = 0;  // tile bit of thread bit u
+        for (uint32_t b = 0; b < B; ++b)
+          if (!(cur_rs & (1u << b))) tp[nt++] = b;
+        const uint32_t NR = 1u << R;
+        std::vector<cd> blk(NR + 32, cd(1.0, 0.0));
+        std::vector<std::pair<uint32_t, std::pair<cd, cd>>> fixed;
+        for (uint32_t m = 0; m < op.npairs; ++m) {
+          const uint32_t q = static_cast<uint32_t>(pool[op.off + 3 * m].real());
+          const cd u = pool[op.off + 3 * m + 1], v = pool[op.off + 3 * m + 2];
+          const int lb = loc[q];
+          if (lb < 0) {
+            fixed.push_back({q, {u, v}});
+          } else if (cur_rs & (1u << lb)) {
+            for (uint32_t j = 0; j < NR; ++j) blk[j] *= ((reg_tile_bits(j) >> lb) & 1u) ? v : u;
+          } else {
+            uint32_t tu = 0;
+            while (tp[tu] != static_cast<uint32_t>(lb)) ++tu;
+            cd *lt = blk.data() + NR + (tu < 4 ? 0 : 16);
+            const uint32_t sh = tu & 3u;
+            for (uint32_t i = 0; i < 16; ++i) lt[i] *= ((i >> sh) & 1u) ? v : u;
+          }
+        }
+        const uint32_t nchunk = static_cast<uint32_t>((fixed.size() + 3) / 4);
+        if (nchunk > 8) throw std::logic_error("aerhip: fused diagonal with too many fixed qubits");
+        std::memset(r.u.fq, 0xFF, sizeof(r.u.fq));
+        for (uint32_t c = 0; c < nchunk; ++c) {
+          cd tab[16];
+          for (cd &t : tab) t = cd(1.0, 0.0);
+          for (uint32_t w = 0; w < 4 && 4 * c + w < fixed.size(); ++w) {
+            const auto &fm = fixed[4 * c + w];
+            r.u.fq[4 * c + w] = static_cast<uint8_t>(fm.first);
+            for (uint32_t i = 0; i < 16; ++i) tab[i] *= ((i >> w) & 1u) ? fm.second.second : fm.second.first;
+          }
+          blk.insert(blk.end(), tab, tab + 16);
+        }
+        r.a = nchunk;
+        r.extra = static_cast<uint32_t>(pool.size());
+        pool.insert(pool.end(), blk.begin(), blk.end());
+        break;
+      }
       case K_PROBE: {
         materialize(pend);
         r.kind = Q_PROBE;
@@ -1179,8 +1439,11 @@ void register_program(const std::vector<TileOp> &in, uint32_t B, uint32_t R, uin
       lds_mode = true;
     }
     // LDS mode: run every ready op that cannot run in registers (and ready
-    // diagonals), then return to registers if register gates are pending.
-    const auto idx = scan_items(rem, [&](const RItem &it) { return it.cls != 0; });
+    // diagonals; fused diagonals run only in registers), then return to
+    // registers if register gates are pending.
+    const auto idx = scan_items(rem, [&](const RItem &it) {
+      return it.cls == 2 || (it.cls == 1 && it.op.kind != K_PHASES);
+    });
     if (!idx.empty()) {
       take(idx, rs, true);
       continue;
@@ -1214,14 +1477,16 @@ std::vector<PassPlan> Engine::plan() const {
     while (!remaining.empty()) {
       uint64_t s = lowmask;
       uint64_t blocked = 0;
+      bool nonunitary_blocked = false;  // a deferred op rescales amplitudes
       PassPlan pp;
       pp.solo = false;
       std::vector<uint32_t> rest;
       for (uint32_t idx : remaining) {
         const Prim &p = queue_[idx];
-        if (p.touch & blocked) {
+        if ((p.touch & blocked) || (p.kind == K_PROBE && nonunitary_blocked)) {
           rest.push_back(idx);
           blocked |= p.touch;
+          nonunitary_blocked |= !p.unitary;
           continue;
         }
         const uint64_t ns = s | p.tmask;
@@ -1231,6 +1496,7 @@ std::vector<PassPlan> Engine::plan() const {
         } else {
           rest.push_back(idx);
           blocked |= p.touch;
+          nonunitary_blocked |= !p.unitary;
         }
       }
       // Complete the tile with the lowest free qubits (longer contiguous runs).
@@ -1321,6 +1587,7 @@ void Engine::emit_program(Program &prog) {
       op.kind = p.kind;
       op.off = p.off + pool_base;
       op.npairs = p.npairs;
+      op.nonunitary = p.unitary ? 0u : 1u;
       op.pr = p.pr;
       op.pi = p.pi;
       for (size_t i = 0; i < p.ctrl.size(); ++i) {
@@ -1367,6 +1634,8 @@ void Engine::emit_program(Program &prog) {
     const uint32_t full_bits =
         prec_ == AERHIP_PRECISION_DOUBLE ? kTileBitsC128 : kTileBitsC64;
     if (!pp.solo && reg_bits_ > 0 && ns == full_bits) {
+      if (fusion_enable_ && n_ > fusion_threshold_)
+        pops = fuse_phase_runs(pops, pd.spos, ns, prog.pool, fusion_max_qubit_, &stats_.fused);
       pd.kind = 2;  // register-layout tile pass: op_begin/nops index the ROps
       pd.op_begin = static_cast<uint32_t>(rops.size());
       register_program(pops, ns, reg_bits_, low_bits_, pd.spos, prog.pool, rops, ops, &stats_.gathers);
@@ -1513,13 +1782,17 @@ void Engine::flush() {
         k_tile_lds<float, kTileBitsC64><<<grid, kThreads, 0, stream_>>>(
             static_cast<cx<float> *>(sv_), dpass + i, dops, dpool, dprobes);
     } else {
-      const uint32_t grid = static_cast<uint32_t>(1ull << (n_ - pd.nb));
+      // One workgroup per group up to 2^16 workgroups; beyond that each
+      // workgroup loops over groups (the grid stays far below the launch
+      // limit of 2^32 work-items per dimension).
+      const uint64_t ngroups = 1ull << (n_ - pd.nb);
+      const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>(ngroups, 1ull << 16));
       if (prec_ == AERHIP_PRECISION_DOUBLE)
         k_dense_group<double><<<grid, kThreads, 0, stream_>>>(static_cast<cx<double> *>(sv_),
-                                                              dpass + i, dops, dpool);
+                                                              dpass + i, dops, dpool, ngroups);
       else
         k_dense_group<float><<<grid, kThreads, 0, stream_>>>(static_cast<cx<float> *>(sv_),
-                                                             dpass + i, dops, dpool);
+                                                             dpass + i, dops, dpool, ngroups);
     }
     AERHIP_CHECK(hipGetLastError());
     stats_.launches++;
@@ -1546,6 +1819,14 @@ void Engine::flush() {
     stats_.pass_ms += ms;
     stats_.timed_launches += passes.size();
   }
+}
+
+void Engine::set_fusion(bool enable, uint32_t max_qubit, uint32_t threshold) {
+  if (max_qubit < 2) throw std::invalid_argument("aerhip: fusion_max_qubit must be at least 2");
+  flush();
+  fusion_enable_ = enable;
+  fusion_max_qubit_ = max_qubit;
+  fusion_threshold_ = threshold;
 }
 
 void Engine::set_reg_bits(uint32_t r) {
@@ -1606,6 +1887,87 @@ double Engine::norm() {
   flush();
   AERHIP_CHECK(hipSetDevice(device_));
   return prec_ == AERHIP_PRECISION_DOUBLE ? norm_t<double>() : norm_t<float>();
+}
+
+template <typename T>
+double Engine::norm_matrix_t(const MarginalDesc &md, const std::vector<cd> &mat) {
+  const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>(md.inner, 4096));
+  const size_t mat_off = 256;
+  char *sc = static_cast<char *>(scratch(mat_off + mat.size() * sizeof(cd)));
+  AERHIP_CHECK(hipMemcpyAsync(sc, &md, sizeof(md), hipMemcpyHostToDevice, stream_));
+  AERHIP_CHECK(hipMemcpyAsync(sc + mat_off, mat.data(), mat.size() * sizeof(cd),
+                              hipMemcpyHostToDevice, stream_));
+  ensure_partial(grid + 1);
+  k_norm_group<T><<<grid, kThreads, 0, stream_>>>(static_cast<const cx<T> *>(sv_),
+                                                  reinterpret_cast<const MarginalDesc *>(sc),
+                                                  reinterpret_cast<const cx<double> *>(sc + mat_off),
+                                                  d_partial_);
+  k_sum_rows<<<1, kThreads, 0, stream_>>>(d_partial_, 1, grid, d_partial_ + grid);
+  AERHIP_CHECK(hipGetLastError());
+  return fetch_scalar(d_partial_ + grid);
+}
+
+// QubitVector::norm(qubits, mat) (qubitvector.hpp:1889-2054) and, with a
+// diagonal written as a matrix, norm_diagonal (qubitvector.hpp:1946-2082).
+double Engine::norm_matrix(const std::vector<uint32_t> &qubits, const std::vector<cd> &mat) {
+  check_qubits(qubits);
+  const size_t k = qubits.size();
+  if (k == 0) return norm();
+  if (k > 10) throw std::invalid_argument("aerhip: norm of a matrix on more than 10 qubits");
+  const uint64_t D = 1ull << k;
+  if (mat.size() != D * D)
+    throw std::invalid_argument("QubitVector: matrix size does not match the number of qubits");
+  flush();
+  AERHIP_CHECK(hipSetDevice(device_));
+  MarginalDesc md;
+  std::memset(&md, 0, sizeof(md));
+  std::vector<uint32_t> sorted = qubits;
+  std::sort(sorted.begin(), sorted.end());
+  md.nq = static_cast<uint32_t>(k);
+  md.inner = size_ >> k;
+  for (uint32_t i = 0; i < md.nq; ++i) {
+    md.sorted[i] = static_cast<uint8_t>(sorted[i]);
+    md.order[i] = static_cast<uint8_t>(qubits[i]);
+  }
+  return prec_ == AERHIP_PRECISION_DOUBLE ? norm_matrix_t<double>(md, mat)
+                                          : norm_matrix_t<float>(md, mat);
+}
+
+cd Engine::get_state(uint64_t pos) {
+  if (pos >= size_) throw std::invalid_argument("aerhip: amplitude index out of range");
+  flush();
+  AERHIP_CHECK(hipSetDevice(device_));
+  if (prec_ == AERHIP_PRECISION_DOUBLE) {
+    AERHIP_CHECK(hipMemcpyAsync(h_scalar_, static_cast<cx<double> *>(sv_) + pos, 16,
+                                hipMemcpyDeviceToHost, stream_));
+    AERHIP_CHECK(hipStreamSynchronize(stream_));
+    return cd(h_scalar_[0], h_scalar_[1]);
+  }
+  AERHIP_CHECK(hipMemcpyAsync(h_scalar_, static_cast<cx<float> *>(sv_) + pos, 8,
+                              hipMemcpyDeviceToHost, stream_));
+  AERHIP_CHECK(hipStreamSynchronize(stream_));
+  const float *f = reinterpret_cast<const float *>(h_scalar_);
+  return cd(f[0], f[1]);
+}
+
+void Engine::set_state(uint64_t pos, cd value) {
+  if (pos >= size_) throw std::invalid_argument("aerhip: amplitude index out of range");
+  flush();
+  AERHIP_CHECK(hipSetDevice(device_));
+  AERHIP_CHECK(hipStreamSynchronize(stream_));  // the staging word is free
+  if (prec_ == AERHIP_PRECISION_DOUBLE) {
+    h_scalar_[0] = value.real();
+    h_scalar_[1] = value.imag();
+    AERHIP_CHECK(hipMemcpyAsync(static_cast<cx<double> *>(sv_) + pos, h_scalar_, 16,
+                                hipMemcpyHostToDevice, stream_));
+  } else {
+    float *f = reinterpret_cast<float *>(h_scalar_);
+    f[0] = static_cast<float>(value.real());
+    f[1] = static_cast<float>(value.imag());
+    AERHIP_CHECK(hipMemcpyAsync(static_cast<cx<float> *>(sv_) + pos, h_scalar_, 8,
+                                hipMemcpyHostToDevice, stream_));
+  }
+  AERHIP_CHECK(hipStreamSynchronize(stream_));
 }
 
 cd Engine::inner_product(Engine &other) {
@@ -1956,7 +2318,7 @@ void run_batch(Engine *const *engines, uint64_t count, int device, double *value
     }
     for (ROp<double> &r : p.rops) {
       const uint32_t kk = r.kind & ~F_ALL;
-      if (kk == Q_U2) r.extra += pb;
+      if (kk == Q_U2 || kk == Q_DIAGF) r.extra += pb;
       else if (kk == Q_DIAG || kk == Q_LDS) r.extra += ob;
     }
     if (prec == AERHIP_PRECISION_DOUBLE)
@@ -2241,6 +2603,20 @@ int aerhip_qv_pauli_sum_to_upper_half(aerhip_qv *qv, const char *labels, const d
   });
 }
 
+int aerhip_qv_set_fusion(aerhip_qv *qv, int enable, int max_qubit, int threshold) {
+  return guard([&] {
+    if (threshold < 0) throw std::invalid_argument("aerhip: fusion_threshold must be >= 0");
+    E(qv).set_fusion(enable != 0, static_cast<uint32_t>(max_qubit), static_cast<uint32_t>(threshold));
+  });
+}
+
+int aerhip_qv_fused_count(aerhip_qv *qv, uint64_t *out) {
+  return guard([&] {
+    if (!out) throw std::invalid_argument("aerhip: null output");
+    *out = E(qv).stats().fused;
+  });
+}
+
 int aerhip_qv_set_register_bits(aerhip_qv *qv, int reg_bits) {
   return guard([&] { E(qv).set_reg_bits(static_cast<uint32_t>(reg_bits)); });
 }
@@ -2326,6 +2702,41 @@ int aerhip_qv_inner_product(aerhip_qv *a, aerhip_qv *b, double *re, double *im) 
     *re = v.real();
     *im = v.imag();
   });
+}
+
+int aerhip_norm_matrix(aerhip_qv *qv, const uint64_t *qubits, uint64_t nq, const double *mat,
+                       double *out) {
+  return guard([&] {
+    if (!out) throw std::invalid_argument("aerhip: null output");
+    if (nq > 10) throw std::invalid_argument("aerhip: norm of a matrix on more than 10 qubits");
+    *out = E(qv).norm_matrix(qvec(qubits, nq), cvec(mat, 1ull << (2 * nq)));
+  });
+}
+
+int aerhip_norm_diagonal(aerhip_qv *qv, const uint64_t *qubits, uint64_t nq, const double *diag,
+                         double *out) {
+  return guard([&] {
+    if (!out) throw std::invalid_argument("aerhip: null output");
+    if (nq > 10) throw std::invalid_argument("aerhip: norm of a diagonal on more than 10 qubits");
+    const std::vector<cd> d = cvec(diag, 1ull << nq);
+    const uint64_t D = 1ull << nq;
+    std::vector<cd> m(D * D, 0.0);
+    for (uint64_t i = 0; i < D; ++i) m[i + D * i] = d[i];
+    *out = E(qv).norm_matrix(qvec(qubits, nq), m);
+  });
+}
+
+int aerhip_qv_get_state(aerhip_qv *qv, uint64_t pos, double *out2) {
+  return guard([&] {
+    if (!out2) throw std::invalid_argument("aerhip: null output");
+    const cd v = E(qv).get_state(pos);
+    out2[0] = v.real();
+    out2[1] = v.imag();
+  });
+}
+
+int aerhip_qv_set_state(aerhip_qv *qv, uint64_t pos, double re, double im) {
+  return guard([&] { E(qv).set_state(pos, cd(re, im)); });
 }
 
 int aerhip_norm(aerhip_qv *qv, double *out) {

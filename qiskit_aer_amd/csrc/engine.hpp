@@ -52,6 +52,7 @@ struct EngineStats {
   double pass_ms = 0;          // device time of the pass kernels (event timing on)
   uint64_t timed_launches = 0; // launches covered by pass_ms
   uint64_t gathers = 0;        // register layout changes in tile passes
+  uint64_t fused = 0;          // gate primitives merged into fused diagonals
 };
 
 class Engine {
@@ -86,6 +87,9 @@ class Engine {
   void set_low_bits(uint32_t c);
   void set_event_timing(bool on);
   void set_reg_bits(uint32_t r);
+  // Engine-side fusion of diagonal runs (Aer's fusion options): on/off, the
+  // most qubits of one fused op, and the qubit count above which it applies.
+  void set_fusion(bool enable, uint32_t max_qubit, uint32_t threshold);
 
   // gates (queued)
   void apply_matrix(const std::vector<uint32_t> &qubits, const std::vector<cd> &mat);
@@ -105,6 +109,11 @@ class Engine {
 
   // readout
   double norm();
+  // QubitVector::norm(qubits, mat): |M psi|^2 with M on `qubits`, not applied.
+  double norm_matrix(const std::vector<uint32_t> &qubits, const std::vector<cd> &mat);
+  // QubitVector::get_state / set_state: one amplitude.
+  cd get_state(uint64_t pos);
+  void set_state(uint64_t pos, cd value);
   double probability(uint64_t outcome);
   std::vector<double> probabilities(const std::vector<uint32_t> &qubits);
   std::vector<uint64_t> sample_measure(const std::vector<double> &rnds);
@@ -146,6 +155,7 @@ class Engine {
   void *scratch(size_t bytes);
   double fetch_scalar(const double *dptr);
   template <typename T> double norm_t();
+  template <typename T> double norm_matrix_t(const MarginalDesc &md, const std::vector<cd> &mat);
   template <typename T> void probabilities_t(const std::vector<uint32_t> &qubits, double *out);
   template <typename T> void sample_t(const std::vector<double> &rnds, uint64_t *out);
   template <typename T> double expval_t(uint64_t xmask, uint64_t zmask, uint32_t xmax, cd ph);
@@ -161,6 +171,9 @@ class Engine {
   uint32_t tile_bits_;  // largest tile (11 for c128, 12 for c64: 32 KiB of LDS)
   uint32_t low_bits_;   // qubits every tile holds (one 128-byte run)
   uint32_t reg_bits_;   // register qubits per thread in full tiles (0: LDS-only passes)
+  bool fusion_enable_ = true;
+  uint32_t fusion_max_qubit_ = 64;
+  uint32_t fusion_threshold_ = 0;
 
   std::vector<Prim> queue_;
   std::vector<cd> pool_;

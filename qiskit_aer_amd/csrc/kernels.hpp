@@ -23,48 +23,57 @@ __device__ __forceinline__ uint64_t insert_zero64(uint64_t p, uint32_t t) {
 }
 
 // ---------------------------------------------------------------------------
-// Dense matrix on 6..10 qubits: one workgroup per group of 2^k amplitudes.
-// The matrix (column-major) is read from global memory; consecutive threads
-// read consecutive rows of one column, so the reads coalesce.
+// Dense matrix on 6..10 qubits: a workgroup takes groups of 2^k amplitudes
+// (group g and every gridDim.x-th after it, 64-bit group index, so the grid
+// stays within the launch limits at any state size).  Controls on qubits
+// outside the group (op.cg / op.cvg) select the groups the matrix applies to
+// (the multiplexer pushes one controlled matrix per control value).  The
+// matrix (column-major) is read from global memory; consecutive threads read
+// consecutive rows of one column, so the reads coalesce.
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_dense_group(
     cx<T> *__restrict__ sv, const PassDesc *__restrict__ pd,
-    const TileOp *__restrict__ ops, const cx<double> *__restrict__ pool) {
+    const TileOp *__restrict__ ops, const cx<double> *__restrict__ pool, uint64_t ngroups) {
   __shared__ cx<T> g[1024];
   const TileOp &op = ops[pd->op_begin];
   const uint32_t k = op.k;
   const uint32_t D = 1u << k;
   const uint32_t tid = threadIdx.x;
-  // Group base: insert zeros at the ascending target positions
-  // (index0, indexes.hpp:183-193).
-  uint64_t base = blockIdx.x;
-  for (uint32_t j = 0; j < pd->nb; ++j) base = insert_zero64(base, pd->spos[j]);
-  for (uint32_t j = tid; j < D; j += kThreads) {
-    uint64_t d = 0;
-    for (uint32_t b = 0; b < k; ++b) d |= (uint64_t)((j >> b) & 1u) << op.gpos[b];
-    g[j] = sv[base + d];
-  }
-  __syncthreads();
-  cx<T> out[4];
-#pragma unroll
-  for (uint32_t r = 0; r < 4; ++r) {
-    const uint32_t i = tid + r * kThreads;
-    if (i < D) {
-      cx<T> acc = mk<T>(0, 0);
-      for (uint32_t j = 0; j < D; ++j)
-        acc = cfma(to_t<T>(pool[op.off + i + (uint64_t)D * j]), g[j], acc);
-      out[r] = acc;
-    }
-  }
-#pragma unroll
-  for (uint32_t r = 0; r < 4; ++r) {
-    const uint32_t i = tid + r * kThreads;
-    if (i < D) {
+  const uint64_t cg = op.cg, cvg = op.cvg;
+  for (uint64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+    // Group base: insert zeros at the ascending target positions
+    // (index0, indexes.hpp:183-193).
+    uint64_t base = grp;
+    for (uint32_t j = 0; j < pd->nb; ++j) base = insert_zero64(base, pd->spos[j]);
+    if ((base & cg) != cvg) continue;  // uniform over the workgroup
+    for (uint32_t j = tid; j < D; j += kThreads) {
       uint64_t d = 0;
-      for (uint32_t b = 0; b < k; ++b) d |= (uint64_t)((i >> b) & 1u) << op.gpos[b];
-      sv[base + d] = out[r];
+      for (uint32_t b = 0; b < k; ++b) d |= (uint64_t)((j >> b) & 1u) << op.gpos[b];
+      g[j] = sv[base + d];
     }
+    __syncthreads();
+    cx<T> out[4];
+#pragma unroll
+    for (uint32_t r = 0; r < 4; ++r) {
+      const uint32_t i = tid + r * kThreads;
+      if (i < D) {
+        cx<T> acc = mk<T>(0, 0);
+        for (uint32_t j = 0; j < D; ++j)
+          acc = cfma(to_t<T>(pool[op.off + i + (uint64_t)D * j]), g[j], acc);
+        out[r] = acc;
+      }
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < 4; ++r) {
+      const uint32_t i = tid + r * kThreads;
+      if (i < D) {
+        uint64_t d = 0;
+        for (uint32_t b = 0; b < k; ++b) d |= (uint64_t)((i >> b) & 1u) << op.gpos[b];
+        sv[base + d] = out[r];
+      }
+    }
+    __syncthreads();  // g is refilled for the next group
   }
 }
 
@@ -202,6 +211,83 @@ struct MarginalDesc {
   uint8_t sorted[64];  // measured qubits ascending
   uint8_t order[64];   // measured qubits in outcome-bit order
 };
+
+// QubitVector::initialize_component (qubitvector.hpp:879-896) on k > 5
+// qubits, in two sweeps so that no group reads an amplitude another thread has
+// already overwritten: first cache[g] = psi[index0(g)] for every group g (the
+// amplitude whose measured bits are all zero), then psi[idx] = cache[g(idx)] *
+// state[i(idx)] with i(idx) the bits of idx on md->order (bit j = qubits[j]).
+template <typename T>
+__global__ void k_gather_group0(const cx<T> *__restrict__ sv, const MarginalDesc *__restrict__ md,
+                                cx<T> *__restrict__ cache) {
+  const uint32_t nq = md->nq;
+  for (uint64_t g = blockIdx.x * (uint64_t)kThreads + threadIdx.x; g < md->inner;
+       g += (uint64_t)gridDim.x * kThreads) {
+    uint64_t idx = g;
+    for (uint32_t j = 0; j < nq; ++j) idx = insert_zero64(idx, md->sorted[j]);
+    cache[g] = sv[idx];
+  }
+}
+
+template <typename T>
+__global__ void k_init_component(cx<T> *__restrict__ sv, const MarginalDesc *__restrict__ md,
+                                 uint64_t size, const cx<T> *__restrict__ cache,
+                                 const cx<double> *__restrict__ state) {
+  const uint32_t nq = md->nq;
+  for (uint64_t idx = blockIdx.x * (uint64_t)kThreads + threadIdx.x; idx < size;
+       idx += (uint64_t)gridDim.x * kThreads) {
+    uint64_t i = 0;
+    for (uint32_t j = 0; j < nq; ++j) i |= ((idx >> md->order[j]) & 1ull) << j;
+    // group index: idx with the measured bits removed (highest first)
+    uint64_t g = idx;
+    for (int j = static_cast<int>(nq) - 1; j >= 0; --j) {
+      const uint32_t q = md->sorted[j];
+      g = ((g >> (q + 1)) << q) | (g & ((1ull << q) - 1ull));
+    }
+    sv[idx] = cmul(cache[g], to_t<T>(state[i]));
+  }
+}
+
+// QubitVector::norm(qubits, mat) (qubitvector.hpp:1889-2054): sum over the
+// groups of |M psi_group|^2, M (D x D, column-major, D = 2^k <= 1024) from the
+// pool, products in the state precision, accumulated in FP64.  Workgroup w
+// takes groups w, w + gridDim.x, ... and writes its partial to partial[w].
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_norm_group(const cx<T> *__restrict__ sv,
+                                                         const MarginalDesc *__restrict__ md,
+                                                         const cx<double> *__restrict__ mat,
+                                                         double *__restrict__ partial) {
+  __shared__ cx<T> g[1024];
+  __shared__ double red[kThreads / 64];
+  const uint32_t k = md->nq;
+  const uint32_t D = 1u << k;
+  const uint32_t tid = threadIdx.x;
+  double acc = 0;
+  for (uint64_t grp = blockIdx.x; grp < md->inner; grp += gridDim.x) {
+    uint64_t base = grp;
+    for (uint32_t j = 0; j < k; ++j) base = insert_zero64(base, md->sorted[j]);
+    for (uint32_t j = tid; j < D; j += kThreads) {
+      uint64_t d = 0;
+      for (uint32_t b = 0; b < k; ++b) d |= (uint64_t)((j >> b) & 1u) << md->order[b];
+      g[j] = sv[base + d];
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < D; i += kThreads) {
+      cx<T> v = mk<T>(0, 0);
+      for (uint32_t j = 0; j < D; ++j) v = cfma(to_t<T>(mat[i + (uint64_t)D * j]), g[j], v);
+      acc += static_cast<double>(v.x) * v.x + static_cast<double>(v.y) * v.y;
+    }
+    __syncthreads();
+  }
+  acc = wave_sum(acc);
+  if ((tid & 63) == 0) red[tid >> 6] = acc;
+  __syncthreads();
+  if (tid == 0) {
+    double s = 0;
+    for (int w = 0; w < kThreads / 64; ++w) s += red[w];
+    partial[blockIdx.x] = s;
+  }
+}
 
 // Block (c, m) sums |psi|^2 over inner indices k in chunk c for outcome m.
 template <typename T>

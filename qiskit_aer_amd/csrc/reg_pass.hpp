@@ -50,6 +50,11 @@ enum RegPassOpKind : uint32_t {
                     // the k-th pair's lower register holds the |1> amplitude
   Q_U1X = 58,       // Q_U1 with an X on the same bit folded in where (base & cg) == cvg
   Q_U1MX = 59,      // Q_U1M with an X on the same logical bit folded in likewise
+  Q_DIAGF = 60,     // fused diagonal: where (l & cl) == cvl (registers: bit j of b), the
+                    // product over its member qubits of (bit ? v : u), from host tables at
+                    // pool[extra..]: NR register factors, 16 + 16 lane factors (thread bits
+                    // 0-3, 4-7), then a = number of 16-entry tables over fixed qubits
+                    // u.fq[4c .. 4c+3] (0xFF: unused)
 };
 // Flag on kind: the op carries a new register relabelling (map_bm, map_t0).
 constexpr uint32_t F_MAP = 0x80000000u;
@@ -63,7 +68,7 @@ constexpr uint32_t F_ALL = F_MAP;
 constexpr bool lean_kind(uint32_t kind) {
   const uint32_t k = kind & ~F_ALL;
   return k == Q_LOAD || k == Q_STORE || k == Q_RELAYOUT || k == Q_U1 || k == Q_U1X || k == Q_U1M ||
-         k == Q_U1MX || k == Q_X || k == Q_D1 || k == Q_PHASE || k == Q_NOP;
+         k == Q_U1MX || k == Q_X || k == Q_D1 || k == Q_PHASE || k == Q_NOP || k == Q_DIAGF;
 }
 
 // Bit positions packed four to a word (byte i of word w is entry 4w+i), so
@@ -107,6 +112,7 @@ struct alignas(128) ROp {
                      // (re a0', im a0', re a1', im a1'), columns over (re a0, im a0, re a1, im a1)
     RLayout lay;
     RProbe pr;
+    uint8_t fq[32];  // Q_DIAGF: fixed qubits of the tables, four per table
   } u;
 };
 static_assert(sizeof(ROp<double>) == 128 && sizeof(ROp<float>) == 128, "ROp is one 128-byte record");
@@ -884,6 +890,42 @@ __device__ __forceinline__ Amps<T, 1 << R> rw_op(const ROp<T> &op, const OpHead<
   if (FULL && kind == Q_U2) return d_u2<T, NR>(ra * reg_bits_of<NR>() + rb, a, tlc, st.ri, cl, cvl, pool + extra);
   const cx<T> one = mk<T>(1, 0);
   Amps<T, NR> o;
+  if (kind == Q_DIAGF) {
+    // Fused diagonal (host tables for this layout): the factor of register j
+    // on this lane is G * L * R_j, G over the qubits fixed by the tile
+    // (uniform), L over the thread bits (per lane), R_j over the register
+    // bits (uniform per register).
+    if (cvl & ~cl) return pass_through(a);  // a tile-level control is not met
+    uint32_t rbits = 0;
+#pragma unroll
+    for (int s = 0; s < R; ++s) rbits |= st.ri.bm[s];
+    const uint32_t clt = cl & ~rbits;
+    const bool lane_ok = (st.tl & clt) == (cvl & clt);
+    const cx<double> *blk = pool + extra;
+    cx<double> g = blk[NR + (tid & 15u)];
+    g = cmul(g, blk[NR + 16 + ((tid >> 4) & 15u)]);
+    const cx<double> *tab = blk + NR + 32;
+    for (uint32_t c = 0; c < ra; ++c, tab += 16) {
+      uint32_t idx = 0;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const uint32_t q = op.u.fq[4 * c + i];
+        if (q < 64) idx |= static_cast<uint32_t>((base >> q) & 1ull) << i;
+      }
+      g = cmul(g, tab[idx]);
+    }
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+      if ((rb >> j) & 1u) {
+        asm volatile("");  // keeps the branch (no if-conversion into selects)
+        const cx<T> f = to_t<T>(cmul(g, blk[j]));
+        o.v[j] = cmul(a.v[j], csel(lane_ok, f, one));
+      } else {
+        o.v[j] = a.v[j];
+      }
+    }
+    return o;
+  }
   // Diagonal kinds.  The predicate (l & cl) == cvl, l = tl | at(j), splits
   // into a lane part (thread tile bits) and a register part (the register
   // tile bits rbits), which is uniform per register j.

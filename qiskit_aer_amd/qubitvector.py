@@ -126,6 +126,16 @@ class QubitVector:
         L.check(self._L.aerhip_qv_stats(self._h, out))
         return {"passes": out[0], "launches": out[1], "gates": out[2], "gathers": out[3]}
 
+    def set_fusion(self, enable: bool = True, max_qubit: int = 64, threshold: int = 0) -> None:
+        """Engine-side fusion of diagonal runs (Aer's fusion_enable /
+        fusion_max_qubit / fusion_threshold)."""
+        L.check(self._L.aerhip_qv_set_fusion(self._h, int(bool(enable)), int(max_qubit), int(threshold)))
+
+    def fused_count(self) -> int:
+        out = C.c_uint64()
+        L.check(self._L.aerhip_qv_fused_count(self._h, C.byref(out)))
+        return out.value
+
     def set_register_bits(self, reg_bits: int) -> None:
         L.check(self._L.aerhip_qv_set_register_bits(self._h, int(reg_bits)))
 
@@ -198,6 +208,33 @@ class QubitVector:
         out = C.c_double()
         L.check(self._L.aerhip_norm(self._h, C.byref(out)))
         return out.value
+
+    def norm_matrix(self, qubits, mat) -> float:
+        """|M psi|^2 for M on `qubits`, not applied (QubitVector::norm(qubits, mat))."""
+        a, qp, n = _q(qubits)
+        m = _colmajor(mat)
+        out = C.c_double()
+        L.check(self._L.aerhip_norm_matrix(self._h, qp, n, m.ctypes.data_as(C.POINTER(C.c_double)),
+                                           C.byref(out)))
+        return out.value
+
+    def norm_diagonal(self, qubits, diag) -> float:
+        """|D psi|^2 for the diagonal D on `qubits` (QubitVector::norm_diagonal)."""
+        a, qp, n = _q(qubits)
+        d = _c(diag)
+        out = C.c_double()
+        L.check(self._L.aerhip_norm_diagonal(self._h, qp, n, d.ctypes.data_as(C.POINTER(C.c_double)),
+                                             C.byref(out)))
+        return out.value
+
+    def get_state(self, pos: int) -> complex:
+        out = np.zeros(2)
+        L.check(self._L.aerhip_qv_get_state(self._h, int(pos), out.ctypes.data_as(C.POINTER(C.c_double))))
+        return complex(out[0], out[1])
+
+    def set_state(self, pos: int, value: complex) -> None:
+        value = complex(value)
+        L.check(self._L.aerhip_qv_set_state(self._h, int(pos), value.real, value.imag))
 
     def inner_product(self, other: "QubitVector") -> complex:
         """<self|other> accumulated in FP64 (the states may differ in precision)."""

@@ -55,20 +55,34 @@ def test_no_device_means_error_not_fallback():
         QubitVector(4)
 
 
-def test_cpp_adaptor_compiles_and_links(tmp_path):
-    """include/qubitvector_hip.hpp (the QubitVector drop-in) builds against the ABI."""
+def _build_cpp(tmp_path, src):
     import shutil
     import subprocess
     if not shutil.which("g++"):
         pytest.skip("g++ not available")
     root = HEADER.parent.parent
+    out = tmp_path / "a.out"
+    return subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", f"-I{root / 'include'}", str(src),
+                           f"-L{L.LIB_PATH.parent}", "-laerhip", f"-Wl,-rpath,{L.LIB_PATH.parent}",
+                           "-o", str(out)], capture_output=True, text=True)
+
+
+def test_cpp_adaptor_compiles_and_links(tmp_path):
+    """include/qubitvector_hip.hpp (the QubitVector drop-in) builds against the ABI."""
     src = tmp_path / "a.cpp"
     src.write_text('#include "qubitvector_hip.hpp"\n'
                    "template class aerhip::QubitVectorHip<double>;\n"
                    "template class aerhip::QubitVectorHip<float>;\n"
                    "int main() { return 0; }\n")
-    out = tmp_path / "a.out"
-    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", f"-I{root / 'include'}", str(src),
-                        f"-L{L.LIB_PATH.parent}", "-laerhip", f"-Wl,-rpath,{L.LIB_PATH.parent}",
-                        "-o", str(out)], capture_output=True, text=True)
+    r = _build_cpp(tmp_path, src)
+    assert r.returncode == 0, r.stderr
+
+
+def test_cpp_adaptor_takes_every_state_call(tmp_path):
+    """Every call Statevector::State makes on its qreg_ (statevector_state.hpp),
+    with the reference's argument types (QV::Rotation enum, AER::Vector
+    results, reg_t / cvector_t arguments), compiles against the adaptor and
+    links against libaerhip.so (tests/adaptor/state_calls.cpp)."""
+    src = pathlib.Path(__file__).resolve().parent / "adaptor" / "state_calls.cpp"
+    r = _build_cpp(tmp_path, src)
     assert r.returncode == 0, r.stderr

@@ -415,3 +415,222 @@ def test_phase_and_diagonal_register_masks(precision):
             e.apply_matrix([a], m); o.apply_matrix([a], m)
     close(e, o, precision)
 
+
+
+def test_multiplexer_many_targets_with_controls():
+    """6 targets with 2 controls runs as a group pass per control value; each
+    applies only where the controls hold their value."""
+    n = 12
+    print("case: multiplexer 6 targets + 2 controls", flush=True)
+    e, o, rng = pair(n, 13)
+    targets, controls = [0, 3, 5, 7, 8, 11], [2, 9]
+    mats = [D.random_unitary(64, rng) for _ in range(4)]
+    stacked = np.vstack(mats)
+    e.apply_multiplexer(controls, targets, stacked)
+    o.apply_multiplexer(controls, targets, stacked)
+    close(e, o)
+    st = D.random_state(7, rng)
+    qs = [4, 10, 1, 6, 9, 0, 11]
+    e.initialize_component(qs, st)
+    o.initialize_component(qs, st)
+    close(e, o)
+
+
+def test_dense_group_pass_at_30_qubits():
+    """A 6-qubit dense matrix on a 30-qubit state launches more groups than one
+    workgroup each could cover: the kernel loops over groups.  The matrix is
+    X on qubit 29 written densely, so the result must equal apply_mcx."""
+    import torch
+    n = 30
+    print("case: dense 6-qubit matrix at 30 qubits", flush=True)
+    if torch.cuda.mem_get_info(0)[0] < 34 * (1 << 30):
+        pytest.skip("needs 34 GiB of free device memory")
+    a, b = engine(n), engine(n)
+    a.initialize_random(77)
+    b.initialize_from_device(a.device_pointer(), 1 << n, "double")
+    qs = [0, 5, 29, 13, 2, 21]
+    M = np.zeros((64, 64))
+    for j in range(64):
+        M[j ^ 4, j] = 1
+    a.apply_mcx([29])
+    b.apply_matrix(qs, M)
+    ov = a.inner_product(b)
+    assert abs(ov - 1) < 1e-12, ov
+    # a multiplexer on the same 6 targets with a control outside them
+    b.apply_multiplexer([7], qs, np.vstack([np.eye(64), M]))
+    a.apply_mcx([7, 29])
+    assert abs(a.inner_product(b) - 1) < 1e-12
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_wide_pauli_with_odd_y_count(precision):
+    """The split form of a Pauli string wider than a tile takes its sign from
+    the index each amplitude moves to, as the reference does: an odd number of
+    Y letters exposes a sign error."""
+    n = 15
+    print(f"case: wide Pauli, odd Y count {precision}", flush=True)
+    e, o, rng = pair(n, 14, precision)
+    qs = list(range(n))
+    for lab in ("Y" * n, "Y" * 3 + "X" * 12):
+        e.apply_pauli(qs, lab, 1j)
+        o.apply_pauli(qs, lab, 1j)
+        close(e, o, precision)
+
+
+def test_probe_not_moved_across_non_unitary_ops():
+    """A saved value queued before an mcphase with |phase| != 1 (or a
+    projector) reads the state before it."""
+    n = 14
+    print("case: probes around non-unitary ops", flush=True)
+    e, o, rng = pair(n, 15)
+    labels = ["ZX", "XY", "IZ"]
+    want = []
+    for lab in labels:
+        e.save_expval_pauli([1, 3], lab)
+        want.append(o.expval_pauli([1, 3], lab))
+    e.apply_mcphase([9, 12], 0.5)
+    o.apply_mcphase([9, 12], 0.5)
+    e.apply_diagonal_matrix([6], [1.0, 0.0])
+    o.apply_diagonal_matrix([6], [1.0, 0.0])
+    for lab in labels:
+        e.save_expval_pauli([1, 3], lab)
+        want.append(o.expval_pauli([1, 3], lab))
+    m = D.random_unitary(2, rng)
+    e.apply_matrix([1], m)
+    o.apply_matrix([1], m)
+    got = e.saved_expvals()
+    assert np.max(np.abs(np.array(got) - np.array(want))) < 1e-12
+    close(e, o)
+
+
+def test_large_measure_reset_initialize_vs_oracle():
+    """Mid-circuit measurement of 17 qubits, reset of 12 and initialize of 7
+    qubits on a 20-qubit state (projectors beyond one diagonal, X moves for the
+    reset, the two-sweep initialize_component)."""
+    from qiskit_aer_amd import AerSimulator
+    n = 20
+    print("case: 20q measure 17 / reset 12 / initialize 7 vs oracle", flush=True)
+    rng = np.random.default_rng(16)
+    c = circuits_layer(n, rng)
+    mq = [int(x) for x in rng.choice(n, 17, replace=False)]
+    c.measure(mq, list(range(17)))
+    c.gate("h", [int(mq[0])])
+    c.reset([int(x) for x in rng.choice(n, 12, replace=False)])
+    for op in circuits_layer(n, rng).ops:
+        c.ops.append(op)
+    c.initialize(D.random_state(7, rng), [int(x) for x in rng.choice(n, 7, replace=False)])
+    c.save_statevector()
+    res = AerSimulator(precision="double").run(c, shots=1, seed_simulator=11)
+    mem_o, sv_o, _ = O.run_circuit(c, shots=1, seed=11)
+    assert int(res.results[0].memory[0]) == int(mem_o[0])
+    assert np.max(np.abs(res.get_statevector() - sv_o)) <= 1e-12
+
+
+def circuits_layer(n, rng):
+    from qiskit_aer_amd.circuit import Circuit
+    c = Circuit(n, num_clbits=17)
+    for q in range(n):
+        c.gate("u3", [q], list(rng.uniform(0, 6.28, 3)))
+    for a in range(0, n - 1, 2):
+        c.gate("cx", [a, a + 1])
+    return c
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_norm_of_matrix_and_diagonal_and_amplitude_access(precision):
+    """norm(qubits, mat) = |M psi|^2 and norm_diagonal against a dense
+    brute force (Kraus probabilities, statevector_state.hpp:1136);
+    get_state / set_state on single amplitudes."""
+    n = 14
+    print(f"case: norm(qubits, mat) {precision}", flush=True)
+    e, o, rng = pair(n, 17, precision)
+    psi = e.copy_to_vector()
+    tol = 1e-12 if precision == "double" else 1e-5
+    for k in (1, 2, 3, 6):
+        qs = [int(x) for x in rng.choice(n, k, replace=False)]
+        m = D.random_unitary(1 << k, rng) * rng.uniform(0.2, 0.9)
+        want = np.linalg.norm(D.apply(psi, m, qs)) ** 2
+        assert abs(e.norm_matrix(qs, m) - want) < tol, k
+        d = rng.standard_normal(1 << k) + 1j * rng.standard_normal(1 << k)
+        want = np.linalg.norm(D.apply(psi, np.diag(d), qs)) ** 2
+        assert abs(e.norm_diagonal(qs, d) - want) < 10 * tol, k
+    for pos in (0, 77, (1 << n) - 1):
+        assert abs(e.get_state(pos) - psi[pos]) < tol
+    e.set_state(123, 0.25 - 0.5j)
+    psi[123] = 0.25 - 0.5j
+    assert np.max(np.abs(e.copy_to_vector() - psi)) < tol
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+@pytest.mark.parametrize("max_qubit", [64, 3])
+def test_fused_diagonal_runs_match_oracle(precision, max_qubit):
+    """Runs of phase gates (cp / cz / p on pairs and single qubits), RZ, S, T
+    and single-qubit projectors between 2x2 gates and saved values, with
+    qubits on register bits, thread bits and qubits fixed by the tile: the
+    engine fuses them into table-driven diagonal ops (or, with max_qubit 3,
+    into short ones) and must give the oracle's state and saved values."""
+    n = 17
+    print(f"case: fused diagonals {precision} max_qubit={max_qubit}", flush=True)
+    e, o, rng = pair(n, 18, precision)
+    e.set_fusion(True, max_qubit, 0)
+    want = []
+    for rnd in range(6):
+        for q in rng.choice(n, 6, replace=False):
+            m = D.random_unitary(2, rng)
+            e.apply_matrix([int(q)], m); o.apply_matrix([int(q)], m)
+        for _ in range(40):
+            kind = rng.integers(0, 6)
+            if kind <= 1:
+                a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+                ph = np.exp(1j * rng.uniform(-3, 3)) if kind == 0 else -1
+                e.apply_mcphase([a, b], ph); o.apply_mcphase([a, b], ph)
+            elif kind == 2:
+                q = int(rng.integers(n))
+                ph = np.exp(1j * rng.uniform(-3, 3))
+                e.apply_mcphase([q], ph); o.apply_mcphase([q], ph)
+            elif kind == 3:
+                q = int(rng.integers(n))
+                t = rng.uniform(-3, 3)
+                e.apply_rotation([q], "z", t); o.apply_rotation([q], "z", t)
+            elif kind == 4:
+                q = int(rng.integers(n))
+                d = [1, np.exp(1j * rng.choice([np.pi / 2, np.pi / 4, -np.pi / 4]))]
+                e.apply_diagonal_matrix([q], d); o.apply_diagonal_matrix([q], d)
+            else:
+                j = int(rng.integers(4, n))
+                for k in rng.choice(j, min(j, 9), replace=False):
+                    th = rng.uniform(-3, 3)
+                    e.apply_mcphase([int(k), j], np.exp(1j * th))
+                    o.apply_mcphase([int(k), j], np.exp(1j * th))
+        if rnd == 3:
+            q = int(rng.integers(n))
+            e.apply_diagonal_matrix([q], [1.0, 0.0]); o.apply_diagonal_matrix([q], [1.0, 0.0])
+        lab = "".join(rng.choice(list("IXYZ"), 4))
+        qs = [int(x) for x in rng.choice(n, 4, replace=False)]
+        e.save_expval_pauli(qs, lab)
+        want.append(o.expval_pauli(qs, lab))
+    got = e.saved_expvals()
+    close(e, o, precision)
+    assert np.max(np.abs(np.array(got) - np.array(want))) < 10 * TOL[precision]
+    assert e.fused_count() > 0
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_qft_fused_vs_unfused_vs_dft(precision):
+    """QFT-18 with fusion on and off: both equal sqrt(N) ifft(psi)."""
+    from qiskit_aer_amd import circuits
+    n = 18
+    print(f"case: QFT fused/unfused {precision}", flush=True)
+    rng = np.random.default_rng(19)
+    psi = D.random_state(n, rng)
+    want = np.sqrt(1 << n) * np.fft.ifft(psi)
+    tol = 1e-12 if precision == "double" else 1e-5
+    for fuse in (True, False):
+        e = engine(n, precision)
+        e.set_fusion(fuse)
+        e.initialize_from_vector(psi)
+        e.apply_ops(circuits.qft(n))
+        assert np.max(np.abs(e.copy_to_vector() - want)) < tol, fuse
+        assert (e.fused_count() > 0) == fuse

@@ -130,6 +130,9 @@ def gate_matrix_via_oracle(name, nq, params):
     ("rxx", "rxx", 2, [0.4]),
     ("ryy", "ryy", 2, [1.4]),
     ("rzx", "rzx", 2, [-0.8]),
+    ("cu", "cu", 2, [0.7, -1.3, 2.1, 0.45]),
+    ("cp", "cphase", 2, [1.25]),
+    ("mcu", "u4", 1, [1.9, 0.3, -0.8, 1.1]),
     ("sx", "sx", 1, []),
     ("sxdg", "sxdg", 1, []),
     ("ecr", "ecr", 2, []),
@@ -147,6 +150,18 @@ def test_gate_matrices_vs_reference_vmatrix(name, ref_name, nq, params):
     # The statevector dispatch builds H as u4(pi/2, 0, pi, 0), not Matrix::H.
     tol = 1e-15 if name != "h" else 2e-16
     assert np.max(np.abs(mine - ref)) <= tol
+
+
+def test_rzz_diagonal_vs_reference_vmatrix():
+    R = ref_or_skip()
+    theta = 0.83
+    out = np.zeros(2 * 16)
+    p = np.array([theta, 0.0, 0.0, 0.0])
+    cnt = R.ref_vmatrix(b"rzz_diag", p.ctypes.data, out.ctypes.data)
+    assert cnt == 4
+    ref = out[0:8:2] + 1j * out[1:8:2]
+    mine = gate_matrix_via_oracle("rzz", 2, [theta])
+    assert np.max(np.abs(mine - np.diag(ref))) <= 1e-15
 
 
 def test_rng_draws_and_sampler_vs_reference_rngengine():
@@ -322,3 +337,163 @@ def test_single_precision_close_to_double():
     _, a, _ = O.run_circuit(c, shots=0, fusion=False)
     _, b, _ = O.run_circuit(c, shots=0, fusion=False, precision="single")
     assert 1 - abs(np.vdot(a, b)) ** 2 < 1e-6
+
+
+# ---- the remaining QubitVector methods: dense brute force / closed forms ----
+
+def test_permutation_matrix_vs_dense():
+    """apply_permutation_matrix (qubitvector.hpp:1350-1434) swaps the listed
+    pairs of local basis states in order: new[i] = old[src[i]]."""
+    rng = np.random.default_rng(31)
+    n = 7
+    for k in (1, 2, 3, 5, 6):
+        qs = [int(x) for x in rng.choice(n, k, replace=False)]
+        d = 1 << k
+        pairs = [tuple(int(x) for x in rng.choice(d, 2, replace=False)) for _ in range(4)]
+        psi = D.random_state(n, rng)
+        v = O.OracleQubitVector(n)
+        v.initialize_from_vector(psi)
+        v.apply_permutation_matrix(qs, pairs)
+        src = list(range(d))
+        for a, b in pairs:
+            src[a], src[b] = src[b], src[a]
+        P = np.zeros((d, d))
+        for i in range(d):
+            P[i, src[i]] = 1
+        assert np.array_equal(v.copy_to_vector(), D.apply(psi, P, qs))
+
+
+def test_multi_swaps_vs_dense():
+    """apply_multi_swaps (qubitvector.hpp:1843-1871): in groups of ten
+    qubits, SWAP of the pairs (q0, q1), (q2, q3), ... of each group."""
+    rng = np.random.default_rng(32)
+    n = 12
+    SW = np.eye(4)[[0, 2, 1, 3]]
+    for qs in ([3, 8, 0, 11], [int(x) for x in rng.permutation(n)]):
+        psi = D.random_state(n, rng)
+        v = O.OracleQubitVector(n)
+        v.initialize_from_vector(psi)
+        v.apply_multi_swaps(qs)
+        ref = psi
+        for g in range(0, len(qs), 10):
+            grp = qs[g:g + 10]
+            for j in range(0, len(grp) - 1, 2):
+                ref = D.apply(ref, SW, [grp[j], grp[j + 1]])
+        assert np.array_equal(v.copy_to_vector(), ref)
+
+
+def test_multiplexer_vs_dense():
+    """apply_multiplexer (qubitvector.hpp:1306-1340): the control value b
+    (bit i = controls[i]) selects the block M_b(i, j) = mat[(b*cols + i) +
+    DIM*j] acting on the targets; as one operator on targets + controls it is
+    block diagonal."""
+    rng = np.random.default_rng(33)
+    n = 7
+    for nt, nc in ((1, 1), (1, 2), (2, 1), (2, 2), (3, 1)):
+        qs = [int(x) for x in rng.choice(n, nt + nc, replace=False)]
+        tq, cq = qs[:nt], qs[nt:]
+        cols, blocks = 1 << nt, 1 << nc
+        mats = [D.random_unitary(cols, rng) for _ in range(blocks)]
+        stacked = np.vstack(mats)  # DIM x cols, block b in rows b*cols ...
+        psi = D.random_state(n, rng)
+        v = O.OracleQubitVector(n)
+        v.initialize_from_vector(psi)
+        v.apply_multiplexer(cq, tq, stacked)
+        full = np.zeros((cols * blocks, cols * blocks), dtype=complex)
+        for b in range(blocks):
+            full[b * cols:(b + 1) * cols, b * cols:(b + 1) * cols] = mats[b]
+        assert np.max(np.abs(v.copy_to_vector() - D.apply(psi, full, tq + cq))) < 1e-14
+
+
+def test_initialize_component_closed_form():
+    """initialize_component (qubitvector.hpp:879-896): within every group,
+    new[i] = old[index0] * state[i] (bit j of i = qubits[j])."""
+    rng = np.random.default_rng(34)
+    n = 8
+    for qs in ([2], [5, 1], [0, 7, 3], [6, 2, 4, 1, 0, 7]):
+        k = len(qs)
+        psi = D.random_state(n, rng)
+        state = D.random_state(k, rng)
+        v = O.OracleQubitVector(n)
+        v.initialize_from_vector(psi)
+        v.initialize_component(qs, state)
+        ref = psi.copy()
+        qmask = sum(1 << q for q in qs)
+        for idx in range(1 << n):
+            i = sum(((idx >> q) & 1) << j for j, q in enumerate(qs))
+            ref[idx] = psi[idx & ~qmask] * state[i]
+        assert np.max(np.abs(v.copy_to_vector() - ref)) < 1e-15
+
+
+def _collapse_case(kind, qubits, seed):
+    n = 5
+    rng = np.random.default_rng(40 + seed)
+    psi = D.random_state(n, rng)
+    c = Circuit(n, num_clbits=len(qubits))
+    c.initialize(psi, list(range(n)))
+    if kind == "measure":
+        c.measure(qubits, list(range(len(qubits))))
+    else:
+        c.reset(qubits)
+    c.save_statevector()
+    mem, sv, _ = O.run_circuit(c, shots=1, seed=seed)
+    return psi, mem, sv
+
+
+def _draw_outcome(psi, qubits, seed):
+    """First RngEngine(seed).rand_int(probs) of the reference (rng.hpp:94-96)."""
+    import ctypes as C
+    R = ref_or_skip()
+    probs = np.zeros(1 << len(qubits))
+    for idx in range(psi.size):
+        probs[sum(((idx >> q) & 1) << j for j, q in enumerate(qubits))] += abs(psi[idx]) ** 2
+    out = np.zeros(1, dtype=np.uint64)
+    R.ref_rng_discrete(seed, probs.ctypes.data, probs.size, 1, out.ctypes.data)
+    return int(out[0]), probs
+
+
+@pytest.mark.parametrize("qubits,seed", [([2], 1), ([1, 3], 2), ([4, 0, 2], 3)])
+def test_measure_collapse_closed_form(qubits, seed):
+    """Post-measurement state = P_m psi / sqrt(p_m) with m the reference's
+    own draw (statevector_state.hpp:936-1014)."""
+    psi, mem, sv = _collapse_case("measure", qubits, seed)
+    m, probs = _draw_outcome(psi, qubits, seed)
+    assert int(mem[0]) == m
+    ref = np.zeros_like(psi)
+    for idx in range(psi.size):
+        if sum(((idx >> q) & 1) << j for j, q in enumerate(qubits)) == m:
+            ref[idx] = psi[idx] / math.sqrt(probs[m])
+    assert np.max(np.abs(sv - ref)) < 1e-15
+
+
+@pytest.mark.parametrize("qubits,seed", [([2], 5), ([1, 3], 6), ([4, 0, 2], 7)])
+def test_reset_closed_form(qubits, seed):
+    """Reset = projection on the drawn outcome m, renormalised, then the basis
+    state m of the reset qubits moved to |0...0> (statevector_state.hpp:
+    952-1011)."""
+    psi, _, sv = _collapse_case("reset", qubits, seed)
+    m, probs = _draw_outcome(psi, qubits, seed)
+    qmask = sum(1 << q for q in qubits)
+    ref = np.zeros_like(psi)
+    for idx in range(psi.size):
+        if sum(((idx >> q) & 1) << j for j, q in enumerate(qubits)) == m:
+            ref[idx & ~qmask] = psi[idx] / math.sqrt(probs[m])
+    assert np.max(np.abs(sv - ref)) < 1e-15
+
+
+def test_reference_transformer_hook_matches_restatement():
+    """The CPU baseline runs the oracle's driver with the reference's own
+    compiled Transformer(AVX2) for matrices and diagonals: it must give the
+    restatement's state (fused brickwork and QFT, 16 qubits, 8 threads)."""
+    ref_or_skip()
+    c = circuits.brickwork_u3_cx(16, 6, seed=3, measure=False)
+    q = circuits.qft(16)
+    for op in q.ops:
+        c.ops.append(op)
+    _, a, _ = O.run_circuit(c, shots=0, threads=8)
+    with O.reference_transformer() as rt:
+        _, b, _ = O.run_circuit(c, shots=0, threads=8)
+    _, a2, _ = O.run_circuit(c, shots=0, threads=8)
+    assert np.max(np.abs(a - b)) < 1e-13
+    assert np.array_equal(a, a2)  # the hook is removed again
+    assert isinstance(rt.avx2, bool)

@@ -76,3 +76,54 @@ def test_global_phase_and_reset():
     res = AerSimulator().run(c, shots=1, seed_simulator=11)
     _, sv_o, _ = O.run_circuit(c, shots=1, seed=11)
     assert np.max(np.abs(res.get_statevector() - sv_o)) <= 1e-12
+
+
+# Every name of the statevector gate set (statevector_state.hpp:314-384):
+# (name, number of qubits or None for a variable count, number of angles).
+GATE_SET = [
+    ("id", 1, 0), ("delay", 1, 1), ("x", 1, 0), ("y", 1, 0), ("z", 1, 0), ("s", 1, 0),
+    ("sdg", 1, 0), ("h", 1, 0), ("t", 1, 0), ("tdg", 1, 0), ("p", 1, 1), ("sx", 1, 0),
+    ("sxdg", 1, 0), ("r", 1, 2), ("rx", 1, 1), ("ry", 1, 1), ("rz", 1, 1), ("u1", 1, 1),
+    ("u2", 1, 2), ("u3", 1, 3), ("u", 1, 3), ("U", 1, 3), ("CX", 2, 0), ("cx", 2, 0),
+    ("cy", 2, 0), ("cz", 2, 0), ("cp", 2, 1), ("cu1", 2, 1), ("cu2", 2, 2), ("cu3", 2, 3),
+    ("cu", 2, 4), ("swap", 2, 0), ("rxx", 2, 1), ("ryy", 2, 1), ("rzz", 2, 1), ("rzx", 2, 1),
+    ("csx", 2, 0), ("csxdg", 2, 0), ("ecr", 2, 0), ("crx", 2, 1), ("cry", 2, 1), ("crz", 2, 1),
+    ("ccx", 3, 0), ("ccz", 3, 0), ("cswap", 3, 0), ("mcx", None, 0), ("mcy", None, 0),
+    ("mcz", None, 0), ("mcr", None, 2), ("mcrx", None, 1), ("mcry", None, 1),
+    ("mcrz", None, 1), ("mcphase", None, 1), ("mcp", None, 1), ("mcu1", None, 1),
+    ("mcu2", None, 2), ("mcu3", None, 3), ("mcu", None, 4), ("mcswap", None, 0),
+    ("mcsx", None, 0), ("mcsxdg", None, 0), ("mcx_gray", None, 0), ("pauli", None, 0),
+]
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_every_gate_name_vs_oracle(precision):
+    """Each gate name of the set, with random angles on random qubit orders
+    (several operand counts for the multi-controlled names), inside a
+    generic 13-qubit state, through aerhip_run_circuit against the oracle's
+    runner: complex128 within 1e-12, complex64 within 2e-5."""
+    n = 13
+    print(f"case: every gate name {precision}", flush=True)
+    rng = np.random.default_rng(77)
+    tol = 1e-12 if precision == "double" else 2e-5
+    sim = AerSimulator(precision=precision)
+    for name, nq, npar in GATE_SET:
+        for count in ([nq] if nq else [1, 2, 3, 5]):
+            if name == "mcswap" and count < 2:
+                continue
+            c = Circuit(n)
+            for q in range(n):
+                c.gate("u3", [q], list(rng.uniform(0, 6.28, 3)))
+            for a in range(0, n - 1, 2):
+                c.gate("cx", [a, a + 1])
+            for _ in range(3):
+                qs = [int(x) for x in rng.choice(n, count, replace=False)]
+                if name == "pauli":
+                    c.pauli("".join(rng.choice(list("IXYZ"), count)), qs)
+                else:
+                    c.gate(name, qs, list(rng.uniform(-3.1, 3.1, npar)))
+            c.save_statevector()
+            sv = sim.run(c, shots=0).get_statevector()
+            _, sv_o, _ = O.run_circuit(c, shots=0, seed=0)
+            err = np.max(np.abs(sv - sv_o))
+            assert err <= tol, (name, count, err)
