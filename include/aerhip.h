@@ -67,6 +67,10 @@ int aerhip_device_count(void);
 /* Version string of the engine library. */
 const char *aerhip_version(void);
 
+/* No counterpart in the reference (its GPU executor queries the devices
+ * itself): free and total bytes of device memory on `device`. */
+int aerhip_device_memory(int device, uint64_t *free_out, uint64_t *total_out);
+
 /* ---- lifecycle ---------------------------------------------------------- */
 
 /* QubitVector(num_qubits) + set_num_qubits (qubitvector.hpp:922-932):

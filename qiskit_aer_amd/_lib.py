@@ -74,6 +74,7 @@ _SIGNATURES = {
     "aerhip_last_error": (C.c_char_p, []),
     "aerhip_device_count": (C.c_int, []),
     "aerhip_version": (C.c_char_p, []),
+    "aerhip_device_memory": (C.c_int, [C.c_int, _u64p, _u64p]),
     "aerhip_qv_create": (C.c_void_p, [C.c_uint64, C.c_int, C.c_int]),
     "aerhip_qv_destroy": (None, [C.c_void_p]),
     "aerhip_qv_num_qubits": (C.c_int, [C.c_void_p, _u64p]),

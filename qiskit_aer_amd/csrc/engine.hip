@@ -2453,6 +2453,21 @@ int aerhip_device_count(void) {
   return n;
 }
 
+int aerhip_device_memory(int device, uint64_t *free_out, uint64_t *total_out) {
+  return guard([&] {
+    if (!free_out || !total_out) throw std::invalid_argument("aerhip: null output");
+    int cur = 0;
+    AERHIP_CHECK(hipGetDevice(&cur));
+    AERHIP_CHECK(hipSetDevice(device));
+    size_t f = 0, t = 0;
+    const hipError_t e = hipMemGetInfo(&f, &t);
+    (void)hipSetDevice(cur);
+    AERHIP_CHECK(e);
+    *free_out = f;
+    *total_out = t;
+  });
+}
+
 const char *aerhip_version(void) { return "aerhip 0.1.0 (gfx950)"; }
 
 // Internal to the library: planning handles and the batch runner used by

@@ -290,6 +290,13 @@ class QubitVector:
         return out.value
 
 
+def device_memory(device: int = 0) -> tuple:
+    """(free, total) bytes of device memory."""
+    free, total = C.c_uint64(), C.c_uint64()
+    L.check(L.lib().aerhip_device_memory(int(device), C.byref(free), C.byref(total)))
+    return free.value, total.value
+
+
 def device_count() -> int:
     """Number of visible HIP devices."""
     return int(L.lib().aerhip_device_count())

@@ -440,10 +440,10 @@ def test_dense_group_pass_at_30_qubits():
     """A 6-qubit dense matrix on a 30-qubit state launches more groups than one
     workgroup each could cover: the kernel loops over groups.  The matrix is
     X on qubit 29 written densely, so the result must equal apply_mcx."""
-    import torch
+    from qiskit_aer_amd.qubitvector import device_memory
     n = 30
     print("case: dense 6-qubit matrix at 30 qubits", flush=True)
-    if torch.cuda.mem_get_info(0)[0] < 34 * (1 << 30):
+    if device_memory(0)[0] < 34 * (1 << 30):
         pytest.skip("needs 34 GiB of free device memory")
     a, b = engine(n), engine(n)
     a.initialize_random(77)

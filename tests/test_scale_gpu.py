@@ -33,9 +33,8 @@ def engine(n, precision="double"):
 
 
 def device_bytes_free():
-    import torch
-    free, _ = torch.cuda.mem_get_info(0)
-    return free
+    from qiskit_aer_amd.qubitvector import device_memory
+    return device_memory(0)[0]
 
 
 def need_device_gib(gib):
