@@ -271,6 +271,20 @@ int aerhip_expval_pauli(aerhip_qv *qv, const uint64_t *qubits, uint64_t nq,
                         const char *pauli, double phase_re, double phase_im,
                         double *out);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): aerhip_expval_pauli on a statevector the caller holds in
+ * device memory (`sv`: 2^num_qubits amplitudes of `precision`, num_qubits
+ * 1..48), with the same kernels and summation order, so the value is
+ * bit-identical.  It allocates nothing: the workgroup partials go to a fixed
+ * device array of 4096 doubles per GPU, held until the process ends.  The
+ * kernels run on `stream` (NULL: the null stream) on the current device and
+ * write the value to `out`, a device pointer; synchronize the stream before
+ * reading it.  Calls that overlap on one GPU share that array and race. */
+int aerhip_expval_pauli_noalloc(const void *sv, uint64_t num_qubits, int precision,
+                                const uint64_t *qubits, uint64_t nq, const char *pauli,
+                                double phase_re, double phase_im, double *out, void *stream);
+
 /* Base::apply_save_expval for one Pauli string (src/simulators/state.hpp:
  * 416-447, evaluated with QubitVector::expval_pauli): queues
  * Re <psi| phase * P |psi> at the current point of the gate sequence, without

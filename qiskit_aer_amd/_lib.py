@@ -118,6 +118,9 @@ _SIGNATURES = {
     "aerhip_probabilities": (C.c_int, [C.c_void_p, _u64p, C.c_uint64, C.c_void_p]),
     "aerhip_sample_measure": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
     "aerhip_expval_pauli": (C.c_int, [C.c_void_p, _u64p, C.c_uint64, C.c_char_p, C.c_double, C.c_double, _dp]),
+    "aerhip_expval_pauli_noalloc": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int, _u64p, C.c_uint64,
+                                              C.c_char_p, C.c_double, C.c_double, C.c_void_p,
+                                              C.c_void_p]),
     "aerhip_save_expval_pauli": (C.c_int, [C.c_void_p, _u64p, C.c_uint64, C.c_char_p, C.c_double,
                                            C.c_double, _u64p]),
     "aerhip_saved_expvals": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, _u64p]),

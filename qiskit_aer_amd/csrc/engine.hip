@@ -2793,3 +2793,96 @@ int aerhip_expval_pauli(aerhip_qv *qv, const uint64_t *qubits, uint64_t nq, cons
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Pauli expectation value that allocates nothing.  Its workgroup partials and
+// nothing else live in `scratch`, a file-scope device array: one copy per GPU,
+// held until the process ends.  Host code never passes `scratch` itself (its
+// host-side name is not a device address): the device address comes from
+// hipGetSymbolAddress, once per device, with that device current.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr uint32_t kNoAllocPartials = 4096;  // workgroup partials: the grid cap of expval_t
+constexpr int kNoAllocMaxDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) double scratch[kNoAllocPartials];
+
+namespace {
+
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+// The device address of `scratch` on the current device (looked up on the
+// first call on that device).
+double *scratch_on_current_device() {
+  int dev = 0;
+  AERHIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocMaxDevices) throw std::runtime_error("aerhip: device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      throw std::runtime_error(std::string("aerhip: scratch symbol address: ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<double *>(p);
+  }
+  return g_scratch_ptr[dev];
+}
+
+}  // namespace
+
+extern "C" {
+
+int aerhip_expval_pauli_noalloc(const void *sv, uint64_t num_qubits, int precision,
+                                const uint64_t *qubits, uint64_t nq, const char *pauli,
+                                double phase_re, double phase_im, double *out, void *stream) {
+  using namespace aerhip;
+  return guard([&] {
+    if (!sv || !out || !pauli || (nq && !qubits)) throw std::invalid_argument("aerhip: null argument");
+    if (precision != AERHIP_PRECISION_DOUBLE && precision != AERHIP_PRECISION_SINGLE)
+      throw std::invalid_argument("aerhip: unknown precision");
+    if (num_qubits < 1 || num_qubits > 48)
+      throw std::invalid_argument("aerhip: expval_pauli_noalloc takes 1 to 48 qubits");
+    if (std::strlen(pauli) != nq) throw std::invalid_argument("aerhip: Pauli label length mismatch");
+    // pauli_masks_and_phase (qubitvector.hpp:2236-2272): character N-1-i acts on qubits[i]
+    uint64_t xm = 0, zm = 0, ny = 0;
+    uint32_t xmax = 0;
+    for (uint64_t i = 0; i < nq; ++i) {
+      if (qubits[i] >= num_qubits) throw std::invalid_argument("aerhip: qubit index out of range");
+      const uint64_t b = 1ull << qubits[i];
+      switch (pauli[nq - 1 - i]) {
+        case 'I': break;
+        case 'X': xm |= b; xmax = std::max(xmax, static_cast<uint32_t>(qubits[i])); break;
+        case 'Z': zm |= b; break;
+        case 'Y':
+          xm |= b; zm |= b; ++ny;
+          xmax = std::max(xmax, static_cast<uint32_t>(qubits[i]));
+          break;
+        default: throw std::invalid_argument("aerhip: invalid Pauli character");
+      }
+    }
+    const cd ph = add_y_phase(ny, cd(phase_re, phase_im));
+    const uint64_t size = 1ull << num_qubits;
+    const uint32_t grid = grid_for(xm ? size / 2 : size, kNoAllocPartials);
+    if (grid > kNoAllocPartials) throw std::logic_error("aerhip: partials exceed the scratch array");
+    double *part = scratch_on_current_device();
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if ((xm | zm) == 0) {
+      // an all-identity label gives the norm (expval_pauli returns norm() there)
+      if (precision == AERHIP_PRECISION_DOUBLE)
+        k_sumsq<double><<<grid, kThreads, 0, st>>>(static_cast<const cx<double> *>(sv), size, part);
+      else
+        k_sumsq<float><<<grid, kThreads, 0, st>>>(static_cast<const cx<float> *>(sv), size, part);
+    } else if (precision == AERHIP_PRECISION_DOUBLE)
+      k_expval_pauli<double><<<grid, kThreads, 0, st>>>(static_cast<const cx<double> *>(sv), size, xm, zm,
+                                                        xmax, ph.real(), ph.imag(), part);
+    else
+      k_expval_pauli<float><<<grid, kThreads, 0, st>>>(static_cast<const cx<float> *>(sv), size, xm, zm,
+                                                       xmax, ph.real(), ph.imag(), part);
+    k_sum_rows<<<1, kThreads, 0, st>>>(part, 1, grid, out);
+    AERHIP_CHECK(hipGetLastError());
+  });
+}
+
+}  // extern "C"
