@@ -399,6 +399,27 @@ int aerhip_batch_pauli_expval(const aerhip_op *ops, const uint64_t *op_offsets,
                               const uint64_t *term_offsets, int precision, int device,
                               int mode, double *energies, double *kernel_ms);
 
+/* ---- parameter bindings -------------------------------------------------- */
+
+/* Energies E_b = <psi_b|H|psi_b>, psi_b = U(theta_b)|0>, of one parametrized
+ * circuit over num_bindings parameter rows.  The reference binds each row
+ * into a copy of the circuit (Op bind_parameter, src/framework/operations.hpp:
+ * 1347-1374) and runs the bound copies as one batch on the GPU
+ * (BatchConverter, src/transpile/batch_converter.hpp:54-157) whose Pauli
+ * expectation values it evaluates together (QubitVectorThrust::
+ * batched_expval_pauli, qubitvector_thrust.hpp:2683-2713).  Here the rows of
+ * one launch group (as many num_qubits statevectors as fit in 70 % of the
+ * free device memory) advance through the circuit together: each tile pass is
+ * one kernel launch for all of them, each row with its own bound matrices.
+ *   ops, param_index, params, labels, coeffs: as aerhip_adjoint_expval_gradient;
+ *   energies     num_bindings values;
+ *   kernel_ms    optional: device time of the pass kernels. */
+int aerhip_bind_pauli_expval(const aerhip_op *ops, uint64_t num_ops, uint64_t num_qubits,
+                             const int64_t *param_index, uint64_t num_params, uint64_t num_bindings,
+                             const double *params, const char *labels, const double *coeffs,
+                             uint64_t nterms, int precision, int device, double *energies,
+                             double *kernel_ms);
+
 /* ---- adjoint differentiation -------------------------------------------- */
 
 /* No counterpart in the reference (which differentiates by re-running
@@ -415,7 +436,9 @@ int aerhip_batch_pauli_expval(const aerhip_op *ops, const uint64_t *op_offsets,
  *   labels       nterms Pauli strings of num_qubits characters (character
  *                num_qubits-1-i acts on qubit i), coeffs their real weights;
  *   energy_out   num_bindings values, grad_out num_bindings x num_params.
- * Uses one (num_qubits+1)-qubit statevector on `device`. */
+ * The bindings run together, as many (num_qubits+1)-qubit statevectors on
+ * `device` as fit in 70 % of its free memory: every pass of the forward and
+ * of the reverse sweep is one kernel launch for all of them. */
 int aerhip_adjoint_expval_gradient(const aerhip_op *ops, uint64_t num_ops, uint64_t num_qubits,
                                    const int64_t *param_index, uint64_t num_params,
                                    uint64_t num_bindings, const double *params,

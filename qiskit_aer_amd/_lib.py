@@ -132,6 +132,9 @@ _SIGNATURES = {
                                                  C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p,
                                                  C.c_char_p, C.c_void_p, C.c_uint64, C.c_int,
                                                  C.c_int, C.c_void_p, C.c_void_p]),
+    "aerhip_bind_pauli_expval": (C.c_int, [C.POINTER(AerhipOp), C.c_uint64, C.c_uint64, C.c_void_p,
+                                           C.c_uint64, C.c_uint64, C.c_void_p, C.c_char_p, C.c_void_p,
+                                           C.c_uint64, C.c_int, C.c_int, C.c_void_p, _dp]),
     "aerhip_run_config_default": (None, [C.POINTER(AerhipRunConfig)]),
     "aerhip_apply_ops": (C.c_int, [C.c_void_p, C.POINTER(AerhipOp), C.c_uint64]),
     "aerhip_run_circuit": (C.c_int, [C.POINTER(AerhipOp), C.c_uint64, C.c_uint64, C.c_uint64,

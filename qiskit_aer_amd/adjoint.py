@@ -52,8 +52,26 @@ class AdjointEstimator:
         del keep
         return energy, grad
 
-    def energy(self, params) -> np.ndarray:
-        """E(theta) rows by forward runs and batched saved expectation values."""
+    def energy(self, params, kernel_ms: list | None = None) -> np.ndarray:
+        """E(theta) for every parameter row, the rows advancing through the
+        circuit together (``aerhip_bind_pauli_expval``: one kernel launch per
+        tile pass for all rows of a launch group)."""
+        p = np.ascontiguousarray(np.asarray(params, dtype=np.float64).reshape(-1, self.num_params))
+        out = np.empty(p.shape[0], dtype=np.float64)
+        ms = C.c_double(0.0)
+        ops, keep = self.circuit.pack()
+        L.check(L.lib().aerhip_bind_pauli_expval(
+            ops, len(self.circuit.ops), self.num_qubits, self.param_index.ctypes.data,
+            self.num_params, p.shape[0], p.ctypes.data, self.labels, self.coeffs.ctypes.data,
+            self.coeffs.size, self.precision, self.device, out.ctypes.data, C.byref(ms)))
+        del keep
+        if kernel_ms is not None:
+            kernel_ms.append(ms.value)
+        return out
+
+    def energy_one_by_one(self, params) -> np.ndarray:
+        """E(theta) rows one binding at a time on one statevector (forward run
+        and saved expectation values per row)."""
         from .qubitvector import QubitVector
         p = np.asarray(params, dtype=np.float64).reshape(-1, self.num_params)
         qv = QubitVector(self.num_qubits, "double" if self.precision == 0 else "single", self.device)
@@ -106,16 +124,24 @@ def _torch_function():
     import torch
 
     class AdjointExpectation(torch.autograd.Function):
-        """E(theta) with dE/dtheta from the adjoint sweep (computed in forward).
+        """E(theta) with dE/dtheta from the adjoint sweep.
 
-        params: (..., num_params) float64 CPU tensor; returns (...) energies."""
+        params: (..., num_params) tensor on any device; returns (...) energies
+        on the same device and dtype.  When no gradient is needed (no_grad,
+        or params that do not require one) only the energies are computed
+        (the bindings' forward runs); otherwise the adjoint sweep computes
+        energy and gradient together in forward."""
 
         @staticmethod
         def forward(ctx, params, estimator):
             p = params.detach().to("cpu", torch.float64).numpy()
-            e, g = estimator.energy_and_gradient(p.reshape(-1, estimator.num_params))
-            ctx.save_for_backward(torch.from_numpy(g.reshape(p.shape)).to(params.dtype))
-            return torch.from_numpy(e.reshape(p.shape[:-1])).to(params.dtype)
+            rows = p.reshape(-1, estimator.num_params)
+            if ctx.needs_input_grad[0]:
+                e, g = estimator.energy_and_gradient(rows)
+                ctx.save_for_backward(torch.from_numpy(g.reshape(p.shape)).to(params.device, params.dtype))
+            else:
+                e = estimator.energy(rows)
+            return torch.from_numpy(e.reshape(p.shape[:-1])).to(params.device, params.dtype)
 
         @staticmethod
         def backward(ctx, grad_out):

@@ -32,6 +32,20 @@ extern "C" __attribute__((visibility("hidden"))) aerhip_qv *aerhip_detail_qv_cre
     uint64_t num_qubits, int precision, int device);
 extern "C" __attribute__((visibility("hidden"))) int aerhip_detail_batch_run(
     aerhip_qv *const *qvs, uint64_t count, int device, double *values, double *kernel_ms);
+// Many states of one size, one planning handle each (engine.hip, MultiState).
+struct aerhip_multi;
+extern "C" __attribute__((visibility("hidden"))) aerhip_multi *aerhip_detail_multi_create(
+    uint64_t num_qubits, int precision, int device, uint64_t count);
+extern "C" __attribute__((visibility("hidden"))) void aerhip_detail_multi_destroy(aerhip_multi *m);
+extern "C" __attribute__((visibility("hidden"))) aerhip_qv *aerhip_detail_multi_planner(aerhip_multi *m,
+                                                                                        uint64_t i);
+extern "C" __attribute__((visibility("hidden"))) int aerhip_detail_multi_initialize(aerhip_multi *m);
+extern "C" __attribute__((visibility("hidden"))) int aerhip_detail_multi_flush(aerhip_multi *m, double *values,
+                                                                               uint64_t capacity,
+                                                                               uint64_t *per_state);
+extern "C" __attribute__((visibility("hidden"))) int aerhip_detail_multi_pauli_sum_to_upper(
+    aerhip_multi *m, const char *labels, const double *coeffs, uint64_t nterms);
+extern "C" __attribute__((visibility("hidden"))) double aerhip_detail_multi_kernel_ms(aerhip_multi *m);
 
 namespace {
 
@@ -315,6 +329,75 @@ std::vector<uint64_t> op_qubits(const aerhip_op &op, uint64_t n) {
   for (uint64_t v : q)
     if (v >= n) throw std::invalid_argument("operation qubit out of range");
   return q;
+}
+
+}  // namespace
+
+namespace {
+
+// A MultiState (many states of one size advancing together) owned here.
+struct Multi {
+  aerhip_multi *m = nullptr;
+  uint64_t count = 0;
+  ~Multi() { reset(); }
+  void reset() {
+    if (m) aerhip_detail_multi_destroy(m);
+    m = nullptr;
+    count = 0;
+  }
+  void make(uint64_t n, int precision, int device, uint64_t cnt) {
+    if (m && count == cnt) return;
+    reset();
+    m = aerhip_detail_multi_create(n, precision, device, cnt);
+    if (!m) throw std::runtime_error(aerhip_last_error());
+    count = cnt;
+  }
+  aerhip_qv *planner(uint64_t i) const { return aerhip_detail_multi_planner(m, i); }
+};
+
+// Bindings that run together: as many n-qubit states as fit in 70 % of the
+// free device memory (at least one), at most `wanted`.
+uint64_t bindings_per_launch(uint64_t n, int precision, int device, uint64_t wanted) {
+  uint64_t free_b = 0, total_b = 0;
+  ck(aerhip_device_memory(device, &free_b, &total_b));
+  const uint64_t bytes = (1ull << n) * (precision == AERHIP_PRECISION_DOUBLE ? 16 : 8);
+  const uint64_t fit = std::max<uint64_t>(1, free_b / 10 * 7 / bytes);
+  return std::min(fit, wanted);
+}
+
+// The circuit bound to row `b` of `params` (Circuit::set_params-style
+// binding of the first angle of each parametrized op, the way the
+// reference's bind_parameter rewrites op params, operations.hpp:1347-1374).
+std::vector<OwnedOp> bind_row(const std::vector<OwnedOp> &base, const int64_t *param_index,
+                              uint64_t num_params, const double *params, uint64_t b) {
+  std::vector<OwnedOp> bound = base;
+  for (size_t i = 0; i < base.size(); ++i)
+    if (param_index[i] >= 0) bound[i].params[0] = params[b * num_params + param_index[i]];
+  return bound;
+}
+
+void queue_ops(aerhip_qv *qv, const std::vector<OwnedOp> &ops) {
+  std::vector<aerhip_op> views(ops.size());
+  for (size_t i = 0; i < ops.size(); ++i) views[i] = ops[i].view();
+  ck(aerhip_apply_ops(qv, views.data(), views.size()));
+}
+
+std::vector<OwnedOp> checked_ops(const aerhip_op *ops, uint64_t num_ops, uint64_t n, const int64_t *param_index,
+                                 uint64_t num_params) {
+  std::vector<OwnedOp> base(num_ops);
+  for (uint64_t i = 0; i < num_ops; ++i) {
+    base[i] = own(ops[i]);
+    for (uint64_t q : base[i].qubits)
+      if (q >= n) throw std::invalid_argument("operation qubit out of range");
+    if (ops[i].type != AERHIP_OP_GATE && ops[i].type != AERHIP_OP_MATRIX &&
+        ops[i].type != AERHIP_OP_DIAGONAL && ops[i].type != AERHIP_OP_BARRIER)
+      throw std::invalid_argument("parameter bindings: only unitary operations are supported");
+    if (param_index[i] >= static_cast<int64_t>(num_params))
+      throw std::invalid_argument("parameter index out of range");
+    if (param_index[i] >= 0 && base[i].params.empty())
+      throw std::invalid_argument("parametrized op without an angle");
+  }
+  return base;
 }
 
 }  // namespace
@@ -634,6 +717,67 @@ int aerhip_batch_pauli_expval(const aerhip_op *ops, const uint64_t *op_offsets,
   }
 }
 
+
+// Energies <psi(theta_b)| H |psi(theta_b)> of one parametrized circuit over
+// many parameter rows, the rows of one launch group advancing through each
+// pass together (one kernel launch per pass for all of them).  The reference
+// binds the parameters of each row into a copy of the circuit
+// (operations.hpp:1347-1374) and on the GPU runs the bound copies as a batch
+// (BatchConverter, src/transpile/batch_converter.hpp:54-157) whose Pauli
+// expectation values it evaluates together (batched_expval_pauli,
+// qubitvector_thrust.hpp:2683-2713).
+int aerhip_bind_pauli_expval(const aerhip_op *ops, uint64_t num_ops, uint64_t num_qubits,
+                             const int64_t *param_index, uint64_t num_params, uint64_t num_bindings,
+                             const double *params, const char *labels, const double *coeffs,
+                             uint64_t nterms, int precision, int device, double *energies,
+                             double *kernel_ms) {
+  try {
+    if (num_ops && (!ops || !param_index)) throw std::invalid_argument("null op list");
+    if (num_bindings && num_params && !params) throw std::invalid_argument("null params");
+    if (nterms && (!labels || !coeffs)) throw std::invalid_argument("null Hamiltonian");
+    if (num_bindings && !energies) throw std::invalid_argument("null output");
+    const uint64_t n = num_qubits;
+    const std::vector<OwnedOp> base = checked_ops(ops, num_ops, n, param_index, num_params);
+    std::vector<uint64_t> all(n);
+    for (uint64_t q = 0; q < n; ++q) all[q] = q;
+    Multi M;
+    double ms = 0;
+    const uint64_t per = num_bindings ? bindings_per_launch(n, precision, device, num_bindings) : 1;
+    for (uint64_t b0 = 0; b0 < num_bindings; b0 += per) {
+      const uint64_t cnt = std::min(per, num_bindings - b0);
+      M.make(n, precision, device, cnt);
+      ck(aerhip_detail_multi_initialize(M.m));
+      aerhip::parallel_for(cnt, [&](uint64_t i) {
+        aerhip_qv *qv = M.planner(i);
+        queue_ops(qv, bind_row(base, param_index, num_params, params, b0 + i));
+        for (uint64_t t = 0; t < nterms; ++t) {
+          const std::string lab(labels + t * n, n);
+          uint64_t slot = 0;
+          ck(aerhip_save_expval_pauli(qv, all.data(), n, lab.c_str(), 1.0, 0.0, &slot));
+        }
+      });
+      std::vector<double> vals(cnt * nterms + 1);
+      uint64_t pps = 0;
+      ck(aerhip_detail_multi_flush(M.m, vals.data(), vals.size(), &pps));
+      for (uint64_t i = 0; i < cnt; ++i) {
+        double e = 0;
+        for (uint64_t t = 0; t < nterms; ++t) e += coeffs[t] * vals[i * pps + t];
+        energies[b0 + i] = e;
+      }
+      ms += aerhip_detail_multi_kernel_ms(M.m);
+      if (cnt != per || b0 + per >= num_bindings) M.reset();
+    }
+    if (kernel_ms) *kernel_ms = ms;
+    return 0;
+  } catch (const std::invalid_argument &e) {
+    aerhip_detail_set_error(e.what());
+    return 2;
+  } catch (const std::exception &e) {
+    aerhip_detail_set_error(e.what());
+    return 1;
+  }
+}
+
 // Energy and gradient of E(theta) = <psi(theta)| H |psi(theta)> by the
 // adjoint method (no counterpart in the reference, which differentiates by
 // re-running circuits, e.g. parameter shift through EstimatorV2).
@@ -646,6 +790,8 @@ int aerhip_batch_pauli_expval(const aerhip_op *ops, const uint64_t *op_offsets,
 //   3. E = <Phi|X_n|Phi>/2;  4. for k = L..1: dE/dtheta_k from
 //      <Phi|Y_n (x) G_k|Phi> (queued, evaluated inside the tile passes), then
 //      apply U_k^dagger to both halves (uncomputation, nothing is stored).
+// The bindings of one launch group run these steps together: each step's
+// passes launch once for all of their states.
 int aerhip_adjoint_expval_gradient(const aerhip_op *ops, uint64_t num_ops, uint64_t num_qubits,
                                    const int64_t *param_index, uint64_t num_params,
                                    uint64_t num_bindings, const double *params,
@@ -657,64 +803,60 @@ int aerhip_adjoint_expval_gradient(const aerhip_op *ops, uint64_t num_ops, uint6
     if (num_bindings && num_params && !params) throw std::invalid_argument("null params");
     if (nterms && (!labels || !coeffs)) throw std::invalid_argument("null Hamiltonian");
     const uint64_t n = num_qubits;
-    std::vector<OwnedOp> base(num_ops);
-    for (uint64_t i = 0; i < num_ops; ++i) {
-      base[i] = own(ops[i]);
-      for (uint64_t q : base[i].qubits)
-        if (q >= n) throw std::invalid_argument("operation qubit out of range");
-      if (ops[i].type != AERHIP_OP_GATE && ops[i].type != AERHIP_OP_MATRIX &&
-          ops[i].type != AERHIP_OP_DIAGONAL && ops[i].type != AERHIP_OP_BARRIER)
-        throw std::invalid_argument("adjoint: only unitary operations are supported");
-      if (param_index[i] >= static_cast<int64_t>(num_params))
-        throw std::invalid_argument("adjoint: parameter index out of range");
-      if (param_index[i] >= 0 && base[i].params.empty())
-        throw std::invalid_argument("adjoint: parametrized op without an angle");
-    }
-    Runner R;
-    R.n = n + 1;
-    R.qv = aerhip_qv_create(n + 1, precision, device);
-    if (!R.qv) throw std::runtime_error(aerhip_last_error());
+    const std::vector<OwnedOp> base = checked_ops(ops, num_ops, n, param_index, num_params);
     const uint64_t flag = n;
-    std::vector<double> vals;
-    for (uint64_t b = 0; b < num_bindings; ++b) {
-      std::vector<OwnedOp> bound = base;
-      for (uint64_t i = 0; i < num_ops; ++i)
-        if (param_index[i] >= 0) bound[i].params[0] = params[b * num_params + param_index[i]];
-      ck(aerhip_qv_initialize(R.qv));
-      for (const OwnedOp &o : bound) {
-        const aerhip_op v = o.view();
-        ck(aerhip_apply_ops(R.qv, &v, 1));
-      }
-      ck(aerhip_qv_pauli_sum_to_upper_half(R.qv, labels, coeffs, nterms));
-      uint64_t slot = 0;
-      ck(aerhip_save_expval_pauli(R.qv, &flag, 1, "X", 1.0, 0.0, &slot));
-      // (slot, parameter, weight) of every gradient probe
-      std::vector<std::tuple<uint64_t, int64_t, double>> probes;
-      for (uint64_t k = num_ops; k-- > 0;) {
-        const OwnedOp &o = bound[k];
-        if (o.type == AERHIP_OP_BARRIER) continue;
-        if (param_index[k] >= 0) {
-          for (const GenTerm &g : generator(o)) {
-            std::vector<uint64_t> q = g.qubits;
-            q.push_back(flag);
-            const std::string lab = "Y" + g.label;  // character 0 acts on the last qubit
-            ck(aerhip_save_expval_pauli(R.qv, q.data(), q.size(), lab.c_str(), 1.0, 0.0, &slot));
-            probes.emplace_back(slot, param_index[k], g.weight);
+    Multi M;
+    const uint64_t per = num_bindings ? bindings_per_launch(n + 1, precision, device, num_bindings) : 1;
+    for (uint64_t b0 = 0; b0 < num_bindings; b0 += per) {
+      const uint64_t cnt = std::min(per, num_bindings - b0);
+      M.make(n + 1, precision, device, cnt);
+      ck(aerhip_detail_multi_initialize(M.m));
+      std::vector<std::vector<OwnedOp>> bound(cnt);
+      aerhip::parallel_for(cnt, [&](uint64_t i) {
+        bound[i] = bind_row(base, param_index, num_params, params, b0 + i);
+        queue_ops(M.planner(i), bound[i]);
+      });
+      uint64_t pps = 0;
+      ck(aerhip_detail_multi_flush(M.m, nullptr, 0, &pps));
+      ck(aerhip_detail_multi_pauli_sum_to_upper(M.m, labels, coeffs, nterms));
+      // (slot, parameter, weight) of every gradient probe, per binding
+      std::vector<std::vector<std::tuple<uint64_t, int64_t, double>>> probes(cnt);
+      aerhip::parallel_for(cnt, [&](uint64_t i) {
+        aerhip_qv *qv = M.planner(i);
+        uint64_t slot = 0;
+        ck(aerhip_save_expval_pauli(qv, &flag, 1, "X", 1.0, 0.0, &slot));
+        std::vector<OwnedOp> rev;
+        for (uint64_t k = num_ops; k-- > 0;) {
+          const OwnedOp &o = bound[i][k];
+          if (o.type == AERHIP_OP_BARRIER) continue;
+          if (param_index[k] >= 0) {
+            queue_ops(qv, rev);
+            rev.clear();
+            for (const GenTerm &g : generator(o)) {
+              std::vector<uint64_t> q = g.qubits;
+              q.push_back(flag);
+              const std::string lab = "Y" + g.label;  // character 0 acts on the last qubit
+              ck(aerhip_save_expval_pauli(qv, q.data(), q.size(), lab.c_str(), 1.0, 0.0, &slot));
+              probes[i].emplace_back(slot, param_index[k], g.weight);
+            }
           }
+          rev.push_back(inverse(o));
         }
-        const OwnedOp inv = inverse(o);
-        const aerhip_op v = inv.view();
-        ck(aerhip_apply_ops(R.qv, &v, 1));
+        queue_ops(qv, rev);
+      });
+      std::vector<double> vals(cnt * (probes[0].size() + 1) + 1);
+      ck(aerhip_detail_multi_flush(M.m, vals.data(), vals.size(), &pps));
+      for (uint64_t i = 0; i < cnt; ++i) {
+        const double *v = vals.data() + i * pps;
+        const uint64_t b = b0 + i;
+        if (energy_out) energy_out[b] = 0.5 * v[0];
+        if (grad_out) {
+          double *g = grad_out + b * num_params;
+          for (uint64_t p = 0; p < num_params; ++p) g[p] = 0.0;
+          for (const auto &t : probes[i]) g[std::get<1>(t)] += std::get<2>(t) * (-0.5) * v[std::get<0>(t)];
+        }
       }
-      vals.assign(probes.size() + 1, 0.0);
-      uint64_t count = 0;
-      ck(aerhip_saved_expvals(R.qv, vals.data(), vals.size(), &count));
-      if (energy_out) energy_out[b] = 0.5 * vals[0];
-      if (grad_out) {
-        double *g = grad_out + b * num_params;
-        for (uint64_t p = 0; p < num_params; ++p) g[p] = 0.0;
-        for (const auto &t : probes) g[std::get<1>(t)] += std::get<2>(t) * (-0.5) * vals[std::get<0>(t)];
-      }
+      if (cnt != per || b0 + per >= num_bindings) M.reset();
     }
     return 0;
   } catch (const std::invalid_argument &e) {

@@ -1658,19 +1658,117 @@ namespace {
 // launch the wide lean variant.
 constexpr uint32_t kWidePassOps = 8;
 
+// The device program of a flush as the pass kernels see it, and the states
+// they run on: `count` states of 2^n amplitudes, state b at sv + b *
+// state_stride amplitudes, taking pass descriptors pd + b * pd_stride and
+// probe partials probes + b * probes_per_state * (tiles of the pass).
+struct DeviceProgram {
+  const PassDesc *passes;
+  const TileOp *ops;
+  const void *rops;
+  const cx<double> *pool;
+  double *probes;
+  uint32_t probes_per_state;
+};
+
 template <typename T, int B, int R>
 void launch_reg_pass(bool lean, bool wide, uint32_t grid, hipStream_t stream, cx<T> *sv, const PassDesc *pd,
-                     const ROp<T> *rops, const TileOp *tops, const cx<double> *pool, double *probes) {
+                     const ROp<T> *rops, const TileOp *tops, const cx<double> *pool, double *probes,
+                     uint64_t state_stride, uint32_t pd_stride, uint32_t tile_log, uint32_t pps) {
   if constexpr (sizeof(T) == 8 && R == 3) {
     if (lean && wide) {
-      k_reg_pass<T, B, R, false, true><<<grid, 1 << (B - R), 0, stream>>>(sv, pd, rops, tops, pool, probes);
+      k_reg_pass<T, B, R, false, true><<<grid, 1 << (B - R), 0, stream>>>(
+          sv, pd, rops, tops, pool, probes, state_stride, pd_stride, tile_log, pps);
       return;
     }
   }
   if (lean)
-    k_reg_pass<T, B, R, false><<<grid, 1 << (B - R), 0, stream>>>(sv, pd, rops, tops, pool, probes);
+    k_reg_pass<T, B, R, false><<<grid, 1 << (B - R), 0, stream>>>(sv, pd, rops, tops, pool, probes,
+                                                                  state_stride, pd_stride, tile_log, pps);
   else
-    k_reg_pass<T, B, R, true><<<grid, 1 << (B - R), 0, stream>>>(sv, pd, rops, tops, pool, probes);
+    k_reg_pass<T, B, R, true><<<grid, 1 << (B - R), 0, stream>>>(sv, pd, rops, tops, pool, probes,
+                                                                 state_stride, pd_stride, tile_log, pps);
+}
+
+// Launches pass index i (host descriptor `pd`, the same pass of every state)
+// over `count` states; `lean` / `wide` pick the register-pass variant.
+void launch_pass(int prec, uint32_t reg_bits, uint32_t n, uint32_t tile_bits, const PassDesc &pd,
+                 uint32_t i, bool lean, bool wide, const DeviceProgram &dp, void *sv, uint64_t count,
+                 uint32_t pd_stride, hipStream_t stream) {
+  const uint64_t state_stride = 1ull << n;
+  const PassDesc *dpd = dp.passes + i;
+  const uint32_t pps = dp.probes_per_state;
+  auto grid_of = [&](uint32_t tile_log) {
+    const uint64_t g = count << tile_log;
+    if (g > 0xFFFFFFFFull) throw std::invalid_argument("aerhip: too many states for one launch");
+    return static_cast<uint32_t>(g);
+  };
+  if (pd.kind == 2) {
+    const uint32_t tl = pd.ncomp;
+    if (prec == AERHIP_PRECISION_DOUBLE) {
+      auto *s = static_cast<cx<double> *>(sv);
+      auto *ro = static_cast<const ROp<double> *>(dp.rops);
+      if (reg_bits == 3)
+        launch_reg_pass<double, kTileBitsC128, 3>(lean, wide, grid_of(tl), stream, s, dpd, ro, dp.ops, dp.pool,
+                                                  dp.probes, state_stride, pd_stride, tl, pps);
+      else
+        launch_reg_pass<double, kTileBitsC128, 4>(lean, wide, grid_of(tl), stream, s, dpd, ro, dp.ops, dp.pool,
+                                                  dp.probes, state_stride, pd_stride, tl, pps);
+    } else {
+      auto *s = static_cast<cx<float> *>(sv);
+      auto *ro = static_cast<const ROp<float> *>(dp.rops);
+      if (reg_bits == 4)
+        launch_reg_pass<float, kTileBitsC64, 4>(lean, wide, grid_of(tl), stream, s, dpd, ro, dp.ops, dp.pool,
+                                                dp.probes, state_stride, pd_stride, tl, pps);
+      else
+        launch_reg_pass<float, kTileBitsC64, 5>(lean, wide, grid_of(tl), stream, s, dpd, ro, dp.ops, dp.pool,
+                                                dp.probes, state_stride, pd_stride, tl, pps);
+    }
+  } else if (pd.kind == 3) {
+    // probe sweep over the whole state, one workgroup per tile-sized slice
+    const uint32_t tl = n - tile_bits;
+    if (prec == AERHIP_PRECISION_DOUBLE)
+      k_probe_sweep<double><<<grid_of(tl), kThreads, 0, stream>>>(
+          static_cast<const cx<double> *>(sv), state_stride, dpd, dp.ops, dp.probes, state_stride, pd_stride,
+          tl, pps);
+    else
+      k_probe_sweep<float><<<grid_of(tl), kThreads, 0, stream>>>(
+          static_cast<const cx<float> *>(sv), state_stride, dpd, dp.ops, dp.probes, state_stride, pd_stride,
+          tl, pps);
+  } else if (pd.kind == 0) {
+    // LDS-resident tile pass (tiles smaller than the register layouts need,
+    // or register layouts switched off).
+    const uint32_t tl = pd.ncomp;
+    if (prec == AERHIP_PRECISION_DOUBLE)
+      k_tile_lds<double, kTileBitsC128><<<grid_of(tl), kThreads, 0, stream>>>(
+          static_cast<cx<double> *>(sv), dpd, dp.ops, dp.pool, dp.probes, state_stride, pd_stride, tl, pps);
+    else
+      k_tile_lds<float, kTileBitsC64><<<grid_of(tl), kThreads, 0, stream>>>(
+          static_cast<cx<float> *>(sv), dpd, dp.ops, dp.pool, dp.probes, state_stride, pd_stride, tl, pps);
+  } else {
+    if (count != 1) throw std::invalid_argument("aerhip: dense matrices on 6 or more qubits run on one state at a time");
+    // One workgroup per group up to 2^16 workgroups; beyond that each
+    // workgroup loops over groups (the grid stays far below the launch
+    // limit of 2^32 work-items per dimension).
+    const uint64_t ngroups = 1ull << (n - pd.nb);
+    const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>(ngroups, 1ull << 16));
+    if (prec == AERHIP_PRECISION_DOUBLE)
+      k_dense_group<double><<<grid, kThreads, 0, stream>>>(static_cast<cx<double> *>(sv), dpd, dp.ops,
+                                                           dp.pool, ngroups);
+    else
+      k_dense_group<float><<<grid, kThreads, 0, stream>>>(static_cast<cx<float> *>(sv), dpd, dp.ops,
+                                                          dp.pool, ngroups);
+  }
+  AERHIP_CHECK(hipGetLastError());
+}
+
+// The register-pass variant of a pass: lean if every op is a lean kind, wide
+// if it also has at most kWidePassOps ops.
+void pass_variant(const PassDesc &pd, const std::vector<ROp<double>> &rops, bool &lean, bool &wide) {
+  lean = true;
+  wide = pd.nops <= kWidePassOps;
+  if (pd.kind != 2) return;
+  for (uint32_t k = 0; k < pd.nops && lean; ++k) lean = lean_kind(rops[pd.op_begin + k].kind);
 }
 }  // namespace
 
@@ -1739,62 +1837,12 @@ void Engine::flush() {
     dprobes = d_probes_;
   }
   if (event_timing_) AERHIP_CHECK(hipEventRecord(t0_ev_, stream_));
+  const DeviceProgram dp{dpass, dops, drops, dpool, dprobes, 0};
   for (size_t i = 0; i < passes.size(); ++i) {
-    const PassDesc &pd = passes[i];
-    if (pd.kind == 2) {
-      const uint32_t grid = static_cast<uint32_t>(1ull << pd.ncomp);
-      bool lean = true;
-      for (uint32_t k = 0; k < pd.nops && lean; ++k) lean = lean_kind(rops[pd.op_begin + k].kind);
-      // short passes (a few gates) are an HBM sweep: more waves per SIMD
-      const bool wide = pd.nops <= kWidePassOps;
-      if (prec_ == AERHIP_PRECISION_DOUBLE) {
-        auto *sv = static_cast<cx<double> *>(sv_);
-        auto *ro = static_cast<const ROp<double> *>(drops);
-        if (reg_bits_ == 3)
-          launch_reg_pass<double, kTileBitsC128, 3>(lean, wide, grid, stream_, sv, dpass + i, ro, dops, dpool, dprobes);
-        else
-          launch_reg_pass<double, kTileBitsC128, 4>(lean, wide, grid, stream_, sv, dpass + i, ro, dops, dpool, dprobes);
-      } else {
-        auto *sv = static_cast<cx<float> *>(sv_);
-        auto *ro = static_cast<const ROp<float> *>(drops);
-        if (reg_bits_ == 4)
-          launch_reg_pass<float, kTileBitsC64, 4>(lean, wide, grid, stream_, sv, dpass + i, ro, dops, dpool, dprobes);
-        else
-          launch_reg_pass<float, kTileBitsC64, 5>(lean, wide, grid, stream_, sv, dpass + i, ro, dops, dpool, dprobes);
-      }
-    } else if (pd.kind == 3) {
-      // probe sweep over the whole state, one workgroup per tile-sized slice
-      const uint32_t grid = static_cast<uint32_t>(1ull << (n_ - tile_bits_));
-      if (prec_ == AERHIP_PRECISION_DOUBLE)
-        k_probe_sweep<double><<<grid, kThreads, 0, stream_>>>(static_cast<const cx<double> *>(sv_),
-                                                              size_, dops + pd.op_begin, dprobes);
-      else
-        k_probe_sweep<float><<<grid, kThreads, 0, stream_>>>(static_cast<const cx<float> *>(sv_),
-                                                             size_, dops + pd.op_begin, dprobes);
-    } else if (pd.kind == 0) {
-      // LDS-resident tile pass (tiles smaller than the register layouts need,
-      // or register layouts switched off).
-      const uint32_t grid = static_cast<uint32_t>(1ull << pd.ncomp);
-      if (prec_ == AERHIP_PRECISION_DOUBLE)
-        k_tile_lds<double, kTileBitsC128><<<grid, kThreads, 0, stream_>>>(
-            static_cast<cx<double> *>(sv_), dpass + i, dops, dpool, dprobes);
-      else
-        k_tile_lds<float, kTileBitsC64><<<grid, kThreads, 0, stream_>>>(
-            static_cast<cx<float> *>(sv_), dpass + i, dops, dpool, dprobes);
-    } else {
-      // One workgroup per group up to 2^16 workgroups; beyond that each
-      // workgroup loops over groups (the grid stays far below the launch
-      // limit of 2^32 work-items per dimension).
-      const uint64_t ngroups = 1ull << (n_ - pd.nb);
-      const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>(ngroups, 1ull << 16));
-      if (prec_ == AERHIP_PRECISION_DOUBLE)
-        k_dense_group<double><<<grid, kThreads, 0, stream_>>>(static_cast<cx<double> *>(sv_),
-                                                              dpass + i, dops, dpool, ngroups);
-      else
-        k_dense_group<float><<<grid, kThreads, 0, stream_>>>(static_cast<cx<float> *>(sv_),
-                                                             dpass + i, dops, dpool, ngroups);
-    }
-    AERHIP_CHECK(hipGetLastError());
+    bool lean = false, wide = false;
+    pass_variant(passes[i], rops, lean, wide);
+    launch_pass(prec_, reg_bits_, n_, tile_bits_, passes[i], static_cast<uint32_t>(i), lean, wide, dp, sv_, 1, 0,
+                stream_);
     stats_.launches++;
   }
   stats_.passes += passes.size();
@@ -2182,12 +2230,12 @@ std::vector<double> Engine::take_saved_expvals() {
   return out;
 }
 
-void Engine::pauli_sum_to_upper(const std::vector<std::string> &labels,
-                                const std::vector<double> &coeffs) {
-  if (n_ < 2) throw std::invalid_argument("aerhip: pauli_sum_to_upper needs two or more qubits");
+std::vector<PauliTerm> Engine::upper_terms(uint32_t n, const std::vector<std::string> &labels,
+                                           const std::vector<double> &coeffs) {
+  if (n < 2) throw std::invalid_argument("aerhip: pauli_sum_to_upper needs two or more qubits");
   if (labels.size() != coeffs.size()) throw std::invalid_argument("aerhip: labels/coeffs size");
-  std::vector<uint32_t> lower(n_ - 1);
-  for (uint32_t q = 0; q + 1 < n_; ++q) lower[q] = q;
+  std::vector<uint32_t> lower(n - 1);
+  for (uint32_t q = 0; q + 1 < n; ++q) lower[q] = q;
   std::vector<PauliTerm> terms;
   for (size_t t = 0; t < labels.size(); ++t) {
     const PauliMasks m = pauli_masks(lower, labels[t]);
@@ -2196,6 +2244,12 @@ void Engine::pauli_sum_to_upper(const std::vector<std::string> &labels,
   }
   std::stable_sort(terms.begin(), terms.end(),
                    [](const PauliTerm &a, const PauliTerm &b) { return a.x < b.x; });
+  return terms;
+}
+
+void Engine::pauli_sum_to_upper(const std::vector<std::string> &labels,
+                                const std::vector<double> &coeffs) {
+  const std::vector<PauliTerm> terms = upper_terms(n_, labels, coeffs);
   flush();
   AERHIP_CHECK(hipSetDevice(device_));
   PauliTerm *dt = static_cast<PauliTerm *>(scratch(std::max<size_t>(1, terms.size()) * sizeof(PauliTerm)));
@@ -2385,6 +2439,235 @@ void run_batch(Engine *const *engines, uint64_t count, int device, double *value
   release();
 }
 
+// ---------------------------------------------------------------------------
+// Many states of one size advancing together.  State b runs the program its
+// planning engine b emits, and each pass index launches once for all states
+// (launch_pass with count > 1: workgroup w takes tile w mod 2^ncomp of state
+// w >> ncomp).  The parameter bindings of one circuit run this way: the
+// reference binds parameters per binding and, on the GPU, batches the bound
+// experiments (BatchConverter, src/transpile/batch_converter.hpp:54-157;
+// batched_expval_pauli, qubitvector_thrust.hpp:2683-2713).
+// ---------------------------------------------------------------------------
+class MultiState {
+ public:
+  MultiState(uint32_t n, int prec, int device, std::vector<Engine *> planners)
+      : n_(n), prec_(prec), device_(device), planners_(std::move(planners)) {
+    if (planners_.empty()) throw std::invalid_argument("aerhip: no states");
+    for (Engine *e : planners_)
+      if (!e->plan_only() || e->num_qubits() != n || e->precision() != prec)
+        throw std::invalid_argument("aerhip: states need planning handles of one size and precision");
+    AERHIP_CHECK(hipSetDevice(device_));
+    AERHIP_CHECK(hipStreamCreate(&st_));
+    AERHIP_CHECK(hipEventCreate(&e0_));
+    AERHIP_CHECK(hipEventCreate(&e1_));
+    AERHIP_CHECK(hipMalloc(&states_, planners_.size() * (1ull << n_) * amp_bytes()));
+  }
+  ~MultiState() {
+    (void)hipSetDevice(device_);
+    if (st_) (void)hipStreamSynchronize(st_);
+    if (states_) (void)hipFree(states_);
+    if (d_prog_) (void)hipFree(d_prog_);
+    if (d_probes_) (void)hipFree(d_probes_);
+    if (d_terms_) (void)hipFree(d_terms_);
+    if (e0_) (void)hipEventDestroy(e0_);
+    if (e1_) (void)hipEventDestroy(e1_);
+    if (st_) (void)hipStreamDestroy(st_);
+  }
+  MultiState(const MultiState &) = delete;
+  MultiState &operator=(const MultiState &) = delete;
+
+  uint64_t count() const { return planners_.size(); }
+  double kernel_ms() const { return kernel_ms_; }
+
+  // |0...0> in every state.
+  void initialize() {
+    AERHIP_CHECK(hipSetDevice(device_));
+    const uint64_t size = 1ull << n_;
+    for (uint64_t b = 0; b < count(); ++b) {
+      if (prec_ == AERHIP_PRECISION_DOUBLE)
+        k_fill_basis<double><<<grid_for(size, 16384), kThreads, 0, st_>>>(state<double>(b), size, 0);
+      else
+        k_fill_basis<float><<<grid_for(size, 16384), kThreads, 0, st_>>>(state<float>(b), size, 0);
+    }
+    AERHIP_CHECK(hipGetLastError());
+  }
+
+  // With the top qubit as a flag, sum_t c_t P_t psi into the flagged half of
+  // every state (Engine::pauli_sum_to_upper on each).
+  void pauli_sum_to_upper(const std::vector<PauliTerm> &terms) {
+    AERHIP_CHECK(hipSetDevice(device_));
+    AERHIP_CHECK(hipStreamSynchronize(st_));
+    if (d_terms_) AERHIP_CHECK(hipFree(d_terms_));
+    d_terms_ = nullptr;
+    AERHIP_CHECK(hipMalloc(&d_terms_, std::max<size_t>(1, terms.size()) * sizeof(PauliTerm)));
+    AERHIP_CHECK(hipMemcpy(d_terms_, terms.data(), terms.size() * sizeof(PauliTerm), hipMemcpyHostToDevice));
+    const uint64_t half = 1ull << (n_ - 1);
+    const auto *dt = static_cast<const PauliTerm *>(d_terms_);
+    const uint32_t nt = static_cast<uint32_t>(terms.size());
+    for (uint64_t b = 0; b < count(); ++b) {
+      if (prec_ == AERHIP_PRECISION_DOUBLE)
+        k_pauli_sum_upper<double><<<grid_for(half, 16384), kThreads, 0, st_>>>(state<double>(b), half, dt, nt);
+      else
+        k_pauli_sum_upper<float><<<grid_for(half, 16384), kThreads, 0, st_>>>(state<float>(b), half, dt, nt);
+    }
+    AERHIP_CHECK(hipGetLastError());
+  }
+
+  // Runs what every planning engine has queued and returns the saved values,
+  // state-major (per_state values of state 0, then of state 1, ...).  States
+  // whose programs have the same pass shapes run together; any state whose
+  // shape differs from state 0's runs on its own.
+  std::vector<double> flush(uint64_t *per_state) {
+    const uint64_t cnt = count();
+    std::vector<Program> parts(cnt);
+    std::vector<uint64_t> np(cnt);
+    parallel_for(cnt, [&](uint64_t i) {
+      np[i] = planners_[i]->probes_queued();
+      planners_[i]->emit_program(parts[i]);
+    });
+    const uint64_t pps = np[0];
+    for (uint64_t i = 1; i < cnt; ++i)
+      if (np[i] != pps) throw std::invalid_argument("aerhip: states save different numbers of values");
+    *per_state = pps;
+    std::vector<double> values(cnt * pps, 0.0);
+    if (pps > 0xFFFFFFFFull) throw std::invalid_argument("aerhip: too many saved values");
+    auto same_shape = [&](const Program &a, const Program &b) {
+      if (a.passes.size() != b.passes.size()) return false;
+      for (size_t k = 0; k < a.passes.size(); ++k)
+        if (a.passes[k].kind != b.passes[k].kind || a.passes[k].ncomp != b.passes[k].ncomp ||
+            a.passes[k].kind == 1)
+          return false;
+      return true;
+    };
+    bool together = true;
+    for (uint64_t i = 1; i < cnt && together; ++i) together = same_shape(parts[0], parts[i]);
+    AERHIP_CHECK(hipSetDevice(device_));
+    AERHIP_CHECK(hipEventRecord(e0_, st_));
+    if (together) {
+      run(parts, 0, cnt, pps, values.data());
+    } else {
+      for (uint64_t i = 0; i < cnt; ++i) run(parts, i, 1, pps, values.data() + i * pps);
+    }
+    AERHIP_CHECK(hipEventRecord(e1_, st_));
+    AERHIP_CHECK(hipStreamSynchronize(st_));
+    float ms = 0;
+    AERHIP_CHECK(hipEventElapsedTime(&ms, e0_, e1_));
+    kernel_ms_ += ms;
+    return values;
+  }
+
+ private:
+  size_t amp_bytes() const { return prec_ == AERHIP_PRECISION_DOUBLE ? 16 : 8; }
+  template <typename T>
+  cx<T> *state(uint64_t b) {
+    return static_cast<cx<T> *>(states_) + b * (1ull << n_);
+  }
+
+  // Runs the programs of states [first, first + cnt) (same pass shapes) and
+  // writes their saved values to values (state-major).
+  void run(const std::vector<Program> &parts, uint64_t first, uint64_t cnt, uint64_t pps, double *values) {
+    const size_t P = parts[first].passes.size();
+    // concatenated program, state-major, indices rebased per state
+    std::vector<PassDesc> passes;
+    std::vector<TileOp> ops;
+    std::vector<ROp<double>> rops;
+    std::vector<cd> pool;
+    std::vector<char> lean(P, 1), wide(P, 1);
+    for (uint64_t b = first; b < first + cnt; ++b) {
+      const Program &p = parts[b];
+      const uint32_t ob = static_cast<uint32_t>(ops.size()), rb = static_cast<uint32_t>(rops.size()),
+                     pb = static_cast<uint32_t>(pool.size());
+      for (size_t k = 0; k < P; ++k) {
+        bool l = false, w = false;
+        pass_variant(p.passes[k], p.rops, l, w);
+        lean[k] = lean[k] && l;
+        wide[k] = wide[k] && w;
+        PassDesc d = p.passes[k];
+        d.op_begin += d.kind == 2 ? rb : ob;
+        passes.push_back(d);
+      }
+      for (TileOp t : p.ops) {
+        t.off += pb;
+        ops.push_back(t);
+      }
+      for (ROp<double> r : p.rops) {
+        const uint32_t kk = r.kind & ~F_ALL;
+        if (kk == Q_U2 || kk == Q_DIAGF) r.extra += pb;
+        else if (kk == Q_DIAG || kk == Q_LDS) r.extra += ob;
+        rops.push_back(r);
+      }
+      pool.insert(pool.end(), p.pool.begin(), p.pool.end());
+      if (pool.size() > 0xFFFFFFFFull || ops.size() > 0xFFFFFFFFull || rops.size() > 0xFFFFFFFFull)
+        throw std::invalid_argument("aerhip: batched program too large");
+    }
+    const size_t pass_bytes = passes.size() * sizeof(PassDesc);
+    const size_t ops_off = align_up(pass_bytes, 256);
+    const size_t rops_off = align_up(ops_off + ops.size() * sizeof(TileOp), 256);
+    const size_t pool_off = align_up(rops_off + rops.size() * sizeof(ROp<double>), 256);
+    const size_t total = pool_off + pool.size() * sizeof(cd) + 16;
+    std::vector<char> h(total);
+    std::memcpy(h.data(), passes.data(), pass_bytes);
+    std::memcpy(h.data() + ops_off, ops.data(), ops.size() * sizeof(TileOp));
+    if (prec_ == AERHIP_PRECISION_DOUBLE)
+      std::memcpy(h.data() + rops_off, rops.data(), rops.size() * sizeof(ROp<double>));
+    else
+      rops_to_float(rops.data(), rops.size(), reinterpret_cast<ROp<float> *>(h.data() + rops_off));
+    if (!pool.empty()) std::memcpy(h.data() + pool_off, pool.data(), pool.size() * sizeof(cd));
+    if (total > prog_cap_) {
+      AERHIP_CHECK(hipStreamSynchronize(st_));
+      if (d_prog_) AERHIP_CHECK(hipFree(d_prog_));
+      d_prog_ = nullptr;
+      prog_cap_ = align_up(total, 1 << 16);
+      AERHIP_CHECK(hipMalloc(&d_prog_, prog_cap_));
+    }
+    // pageable source: the copy is complete when hipMemcpyAsync returns
+    AERHIP_CHECK(hipMemcpyAsync(d_prog_, h.data(), total, hipMemcpyHostToDevice, st_));
+    AERHIP_CHECK(hipStreamSynchronize(st_));
+    const uint32_t tb = planners_[first]->tile_bits();
+    const uint64_t ntiles = 1ull << (n_ - tb);
+    if (pps) {
+      const size_t need = (cnt * pps * ntiles + cnt * pps) * sizeof(double);
+      if (need > probes_cap_) {
+        if (d_probes_) AERHIP_CHECK(hipFree(d_probes_));
+        d_probes_ = nullptr;
+        probes_cap_ = align_up(need, 1 << 20);
+        AERHIP_CHECK(hipMalloc(reinterpret_cast<void **>(&d_probes_), probes_cap_));
+      }
+    }
+    char *d = static_cast<char *>(d_prog_);
+    const DeviceProgram dp{reinterpret_cast<const PassDesc *>(d), reinterpret_cast<const TileOp *>(d + ops_off),
+                           d + rops_off, reinterpret_cast<const cx<double> *>(d + pool_off),
+                           pps ? d_probes_ : nullptr, static_cast<uint32_t>(pps)};
+    void *sv = static_cast<char *>(states_) + first * (1ull << n_) * amp_bytes();
+    for (size_t k = 0; k < P; ++k)
+      launch_pass(prec_, planners_[first]->reg_bits(), n_, tb, parts[first].passes[k],
+                  static_cast<uint32_t>(k), lean[k] != 0, wide[k] != 0, dp, sv, cnt,
+                  static_cast<uint32_t>(P), st_);
+    if (pps) {
+      double *sums = d_probes_ + cnt * pps * ntiles;
+      k_sum_rows<<<grid_for(cnt * pps, 1u << 30), kThreads, 0, st_>>>(d_probes_, cnt * pps,
+                                                                      static_cast<uint32_t>(ntiles), sums);
+      AERHIP_CHECK(hipGetLastError());
+      AERHIP_CHECK(hipMemcpyAsync(values, sums, cnt * pps * sizeof(double), hipMemcpyDeviceToHost, st_));
+    }
+    AERHIP_CHECK(hipStreamSynchronize(st_));
+  }
+
+  uint32_t n_;
+  int prec_;
+  int device_;
+  std::vector<Engine *> planners_;
+  void *states_ = nullptr;
+  hipStream_t st_ = nullptr;
+  hipEvent_t e0_ = nullptr, e1_ = nullptr;
+  void *d_prog_ = nullptr;
+  size_t prog_cap_ = 0;
+  double *d_probes_ = nullptr;
+  size_t probes_cap_ = 0;
+  void *d_terms_ = nullptr;
+  double kernel_ms_ = 0;
+};
+
 }  // namespace aerhip
 
 using aerhip::cd;
@@ -2481,6 +2764,75 @@ __attribute__((visibility("hidden"))) aerhip_qv *aerhip_detail_qv_create_planner
     h = new aerhip_qv{std::move(eng)};
   });
   return rc == 0 ? h : nullptr;
+}
+
+// Many states of one size (aerhip::MultiState) with one planning handle per
+// state, used by the parameter-binding runners (controller.cpp).
+struct aerhip_multi {
+  std::vector<aerhip_qv> planners;
+  std::unique_ptr<aerhip::MultiState> m;
+};
+
+__attribute__((visibility("hidden"))) aerhip_multi *aerhip_detail_multi_create(uint64_t num_qubits,
+                                                                                int precision, int device,
+                                                                                uint64_t count) {
+  aerhip_multi *h = nullptr;
+  const int rc = guard([&] {
+    if (!count) throw std::invalid_argument("aerhip: no states");
+    auto m = std::make_unique<aerhip_multi>();
+    m->planners.resize(count);
+    std::vector<Engine *> engines(count);
+    for (uint64_t i = 0; i < count; ++i) {
+      m->planners[i].eng = std::make_unique<Engine>(static_cast<uint32_t>(num_qubits), precision, device,
+                                                    Engine::PlanOnly{});
+      engines[i] = m->planners[i].eng.get();
+    }
+    m->m = std::make_unique<aerhip::MultiState>(static_cast<uint32_t>(num_qubits), precision, device,
+                                                std::move(engines));
+    h = m.release();
+  });
+  return rc == 0 ? h : nullptr;
+}
+
+__attribute__((visibility("hidden"))) void aerhip_detail_multi_destroy(aerhip_multi *m) { delete m; }
+
+__attribute__((visibility("hidden"))) aerhip_qv *aerhip_detail_multi_planner(aerhip_multi *m, uint64_t i) {
+  if (!m || i >= m->planners.size()) return nullptr;
+  return &m->planners[i];
+}
+
+__attribute__((visibility("hidden"))) int aerhip_detail_multi_initialize(aerhip_multi *m) {
+  return guard([&] {
+    if (!m) throw std::invalid_argument("aerhip: null argument");
+    m->m->initialize();
+  });
+}
+
+__attribute__((visibility("hidden"))) int aerhip_detail_multi_flush(aerhip_multi *m, double *values,
+                                                                    uint64_t capacity, uint64_t *per_state) {
+  return guard([&] {
+    if (!m || !per_state) throw std::invalid_argument("aerhip: null argument");
+    const std::vector<double> v = m->m->flush(per_state);
+    if (v.size() > capacity) throw std::invalid_argument("aerhip: saved values exceed the capacity");
+    if (!v.empty()) std::memcpy(values, v.data(), v.size() * sizeof(double));
+  });
+}
+
+__attribute__((visibility("hidden"))) int aerhip_detail_multi_pauli_sum_to_upper(aerhip_multi *m,
+                                                                                  const char *labels,
+                                                                                  const double *coeffs,
+                                                                                  uint64_t nterms) {
+  return guard([&] {
+    if (!m || (nterms && (!labels || !coeffs))) throw std::invalid_argument("aerhip: null argument");
+    const uint32_t n = m->planners[0].eng->num_qubits();
+    std::vector<std::string> labs(nterms);
+    for (uint64_t t = 0; t < nterms; ++t) labs[t].assign(labels + t * (n - 1), n - 1);
+    m->m->pauli_sum_to_upper(Engine::upper_terms(n, labs, std::vector<double>(coeffs, coeffs + nterms)));
+  });
+}
+
+__attribute__((visibility("hidden"))) double aerhip_detail_multi_kernel_ms(aerhip_multi *m) {
+  return m ? m->m->kernel_ms() : 0.0;
 }
 
 // Runs the queues of `count` planning handles (same precision) as one batch

@@ -131,12 +131,17 @@ class Engine {
   // with the top qubit 0) into the half with the top qubit 1.  Labels act on
   // qubits 0..n-2 (character n-2-i on qubit i).
   void pauli_sum_to_upper(const std::vector<std::string> &labels, const std::vector<double> &coeffs);
+  // The terms of such a Pauli sum on n qubits (flag excluded), sorted by X mask.
+  static std::vector<PauliTerm> upper_terms(uint32_t n, const std::vector<std::string> &labels,
+                                            const std::vector<double> &coeffs);
 
   // Executes the queue.
   void flush();
   // Appends the queue's passes to `prog` and clears the queue.
   void emit_program(Program &prog);
   bool plan_only() const { return plan_only_; }
+  uint32_t tile_bits() const { return tile_bits_; }
+  uint32_t reg_bits() const { return reg_bits_; }
   uint64_t probes_queued() const { return probes_queued_; }
 
  private:

@@ -1117,15 +1117,17 @@ __global__ __launch_bounds__(1 << (B - R))
 __attribute__((amdgpu_waves_per_eu(reg_pass_waves<T, B, R, WIDE>()))) void k_reg_pass(
     cx<T> *__restrict__ sv, const PassDesc *__restrict__ pd, const ROp<T> *__restrict__ rops,
     const TileOp *__restrict__ tops, const cx<double> *__restrict__ pool,
-    double *__restrict__ probes) {
+    double *__restrict__ probes, uint64_t state_stride, uint32_t pd_stride, uint32_t tile_log,
+    uint32_t probes_per_state) {
   static_assert(!WIDE || (!FULL && sizeof(T) == 8), "the wide variant is the lean complex128 one");
   __shared__ cx<T> lds[1 << B];
   __shared__ cx<T> mat[16];
   __shared__ uint32_t doff[32];
   __shared__ double red[(1 << (B - R)) / 64];
-  const uint64_t t = xcd_tile(blockIdx.x, gridDim.x);
-  const ProbeSink sink{probes, t, gridDim.x, false};
-  reg_tile<T, B, R, 16, FULL>(sv, pd, rops, tops, pool, sink, t, lds, mat, doff, red);
+  const StateTile st = state_tile(tile_log);
+  const ProbeSink sink{probes + st.b * probes_per_state * st.ntiles, st.t, st.ntiles, false};
+  reg_tile<T, B, R, 16, FULL>(sv + st.b * state_stride, pd + st.b * pd_stride, rops, tops, pool,
+                              sink, st.t, lds, mat, doff, red);
 }
 
 }  // namespace aerhip

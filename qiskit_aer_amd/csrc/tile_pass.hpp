@@ -352,16 +352,37 @@ __device__ __forceinline__ double probe_sweep_block(const cx<T> *__restrict__ sv
   return block_sum_nt<NT>(acc, red);
 }
 
-// One workgroup per slice: probes[slot * gridDim.x + slice].
+// Workgroup -> (state, tile) of a pass over `count` states of one size that
+// run the same pass index: workgroup w (after the XCD-aware order) takes tile
+// w mod 2^tile_log of state w >> tile_log.  State b's amplitudes start at
+// b * state_stride, its pass descriptor at pd + b * pd_stride, its probe
+// partials at probes + b * probes_per_state * 2^tile_log.  A single state is
+// count 1 (strides unused).
+struct StateTile {
+  uint64_t b, t, ntiles;
+};
+__device__ __forceinline__ StateTile state_tile(uint32_t tile_log) {
+  const uint64_t w = xcd_tile(blockIdx.x, gridDim.x);
+  return StateTile{w >> tile_log, w & ((1ull << tile_log) - 1ull), 1ull << tile_log};
+}
+
+// One workgroup per slice (2^tile_log slices per state):
+// probes[(b * probes_per_state + slot) * 2^tile_log + slice].  The probe of
+// state b is ops[pd_b->op_begin], pd_b = pd + b * pd_stride.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_probe_sweep(const cx<T> *__restrict__ sv,
                                                           uint64_t size,
-                                                          const TileOp *__restrict__ op,
-                                                          double *__restrict__ probes) {
+                                                          const PassDesc *__restrict__ pd,
+                                                          const TileOp *__restrict__ ops,
+                                                          double *__restrict__ probes,
+                                                          uint64_t state_stride, uint32_t pd_stride,
+                                                          uint32_t tile_log, uint32_t probes_per_state) {
   __shared__ double red[kThreads / 64];
-  const uint64_t t = xcd_tile(blockIdx.x, gridDim.x);
-  const double v = probe_sweep_block<T, kThreads>(sv, size, *op, t, gridDim.x, red);
-  if (threadIdx.x == 0) probes[static_cast<uint64_t>(op->npairs) * gridDim.x + t] = v;
+  const StateTile st = state_tile(tile_log);
+  const TileOp &o = ops[pd[st.b * pd_stride].op_begin];
+  const double v = probe_sweep_block<T, kThreads>(sv + st.b * state_stride, size, o, st.t, st.ntiles, red);
+  if (threadIdx.x == 0)
+    probes[(st.b * probes_per_state + o.npairs) * st.ntiles + st.t] = v;
 }
 
 // One workgroup per tile.
@@ -370,16 +391,18 @@ __global__ __launch_bounds__(kThreads) void k_tile_lds(cx<T> *__restrict__ sv,
                                                        const PassDesc *__restrict__ pd,
                                                        const TileOp *__restrict__ ops,
                                                        const cx<double> *__restrict__ pool,
-                                                       double *__restrict__ probes) {
+                                                       double *__restrict__ probes,
+                                                       uint64_t state_stride, uint32_t pd_stride,
+                                                       uint32_t tile_log, uint32_t probes_per_state) {
   __shared__ cx<T> tile[1 << BMAX];
   __shared__ uint64_t hioff[256];  // 2^(nb - c) <= 256: the planner keeps nb - c <= 8
   __shared__ cx<T> mat[1024];
   __shared__ uint32_t doff[32];
   __shared__ double red[kThreads / 64];
-  const uint64_t t = xcd_tile(blockIdx.x, gridDim.x);
-  const ProbeSink sink{probes, t, gridDim.x, false};
-  lds_tile<T, kThreads, BMAX, 1024>(sv, pd, ops, pool, sink, t, tile, hioff, mat, doff,
-                                    red);
+  const StateTile st = state_tile(tile_log);
+  const ProbeSink sink{probes + st.b * probes_per_state * st.ntiles, st.t, st.ntiles, false};
+  lds_tile<T, kThreads, BMAX, 1024>(sv + st.b * state_stride, pd + st.b * pd_stride, ops, pool,
+                                    sink, st.t, tile, hioff, mat, doff, red);
 }
 
 }  // namespace aerhip

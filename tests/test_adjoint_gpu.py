@@ -128,3 +128,47 @@ def test_adjoint_batch_and_torch_autograd():
     assert np.max(np.abs(theta.grad.numpy() - g)) < 1e-12
     ps = est.parameter_shift(rows[:1], which=[0, 5, npar - 1])
     assert np.max(np.abs(ps[0] - g[0, [0, 5, npar - 1]])) < 1e-9
+
+
+@pytest.mark.parametrize("n,precision,tol", [(12, "double", 1e-10), (17, "double", 1e-10),
+                                             (17, "single", 2e-4)])
+def test_bound_rows_in_one_launch_vs_oracle(n, precision, tol):
+    """aerhip_bind_pauli_expval: every row's energy against the oracle and,
+    bit for bit in c128, against the row run on its own.  Row 0 binds every
+    angle to 0 (its rotations are identities, so its program has another
+    shape and runs apart from the rows that share one)."""
+    print(f"case: bound rows n={n} {precision}", flush=True)
+    c, idx, npar, _ = ansatz(n, 2, seed=n + 1)
+    ham = tfim(n)
+    est = AdjointEstimator(c, idx, npar, ham, precision=precision)
+    rows = np.random.default_rng(n).uniform(-math.pi, math.pi, size=(6, npar))
+    rows[0] = 0.0
+    ms = []
+    e = est.energy(rows, kernel_ms=ms)
+    assert ms and ms[0] > 0
+    want = np.array([oracle_energy(c, idx, r, ham) for r in rows])
+    assert np.max(np.abs(e - want)) < tol, (e, want)
+    if precision == "double":
+        one = est.energy_one_by_one(rows)
+        assert np.max(np.abs(e - one)) < 1e-12
+
+
+def test_adjoint_bindings_together_match_one_by_one():
+    """Five bindings of a 15-qubit ansatz (16 qubits with the flag, 32 tiles
+    per state) run together; each row equals its own single-row run."""
+    import torch
+    n = 15
+    print("case: adjoint 5 bindings together, n=15", flush=True)
+    c, idx, npar, _ = ansatz(n, 2, seed=4)
+    ham = tfim(n)
+    est = AdjointEstimator(c, idx, npar, ham)
+    rows = np.random.default_rng(9).uniform(-3, 3, size=(5, npar))
+    e, g = est.energy_and_gradient(rows)
+    for b in (0, 3):
+        e1, g1 = est.energy_and_gradient(rows[b][None, :])
+        assert abs(e[b] - e1[0]) < 1e-12 and np.max(np.abs(g[b] - g1[0])) < 1e-12
+    assert abs(e[2] - oracle_energy(c, idx, rows[2], ham)) < 1e-10
+    theta = torch.tensor(rows, dtype=torch.float64)
+    with torch.no_grad():
+        e_ng = adjoint_expectation(theta, est)
+    assert np.max(np.abs(e_ng.numpy() - e)) < 1e-12
