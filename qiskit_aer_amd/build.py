@@ -73,7 +73,24 @@ def build(verbose: bool = True) -> pathlib.Path:
     tmp = OUT.with_suffix(".so.tmp")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", *objs, "-o", str(tmp)], verbose)
     os.replace(tmp, OUT)
+    build_controller_wrappers(verbose)
     return OUT
+
+
+def build_controller_wrappers(verbose: bool = True) -> pathlib.Path:
+    """The controller_wrappers-compatible pybind11 module, linked against
+    libaerhip.so (found next to it through an $ORIGIN run path)."""
+    import sysconfig
+
+    import pybind11
+    ext = sysconfig.get_config_var("EXT_SUFFIX") or ".so"
+    out = PKG / f"controller_wrappers{ext}"
+    tmp = out.with_name(out.name + ".tmp")
+    _run(["g++", *CXX_FLAGS, "-shared", f"-I{pybind11.get_include()}",
+          f"-I{sysconfig.get_paths()['include']}", str(PKG / "pybind" / "controller_wrappers.cpp"),
+          f"-L{OUT.parent}", "-laerhip", "-Wl,-rpath,$ORIGIN/lib", "-o", str(tmp)], verbose)
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":
