@@ -29,9 +29,9 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X published FP64 peak (vector and matrix)
 
 # HBM bytes per tile-pass launch of the default workload (30 qubits, c128),
 # from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (profiles/reg_pass_30q_brickwork_c128.md):
-# 2.0 x 8 388 793 KiB fetched (FETCH_SIZE counts half the bytes, calibrated on a
-# kernel that reads the 16 GiB state once) + 16 777 476 KiB written.
-MEASURED_TRAFFIC_30Q_C128 = (2 * 8388793 + 16777476) * 1024
+# 2.0 x 8 388 809 KiB fetched (FETCH_SIZE counts half the bytes, calibrated on a
+# kernel that reads the 16 GiB state once) + 16 777 668 KiB written.
+MEASURED_TRAFFIC_30Q_C128 = (2 * 8388809 + 16777668) * 1024
 
 
 def log(msg: str) -> None:
@@ -231,9 +231,10 @@ def cpu_baseline(n: int, layers: int, seed: int) -> dict:
                       f"Aer fusion on (max 5), {path}, gate phase {timing[0]:.2f} s"}
 
 
-def profile_summary(trace_db: str, fetch_db: str = None, write_db: str = None) -> str:
+def profile_summary(trace_db: str, *pmc_dbs: str) -> str:
     """Text summary of rocprofv3 output databases: per-kernel calls and average
-    duration (kernel trace), and FETCH_SIZE / WRITE_SIZE per dispatch (PMC runs)."""
+    duration (kernel trace), and every counter of the PMC runs per dispatch
+    (average over the kernel's dispatches)."""
     import collections
     import sqlite3
     out = []
@@ -244,26 +245,26 @@ def profile_summary(trace_db: str, fetch_db: str = None, write_db: str = None) -
     for name, calls, total, avg, pct in cur.execute(
             "select name, total_calls, total_duration, average, percentage from top_kernels"):
         out.append(f"| `{name.split('(')[0]}` | {calls} | {total / 1e3:.1f} | {avg / 1e3:.3f} | {pct:.2f} |")
-    for label, db, counter in (("FETCH_SIZE", fetch_db, "FETCH_SIZE"), ("WRITE_SIZE", write_db, "WRITE_SIZE")):
+    for db in pmc_dbs:
         if not db:
             continue
         c = sqlite3.connect(db).cursor()
-        agg = collections.defaultdict(list)
-        for name, v in c.execute("select kernel_name, value from counters_collection where counter_name=?",
-                                 (counter,)):
-            agg[name.split("(")[0]].append(v)
-        out.append("")
-        out.append(f"| kernel | dispatches | {label} KiB/dispatch (avg) |")
-        out.append("|---|---|---|")
-        for k, vs in agg.items():
-            out.append(f"| `{k}` | {len(vs)} | {sum(vs) / len(vs):.1f} |")
+        agg = collections.defaultdict(lambda: collections.defaultdict(list))
+        for name, counter, v in c.execute(
+                "select kernel_name, counter_name, value from counters_collection"):
+            agg[counter][name.split("(")[0]].append(v)
+        for counter, per_kernel in sorted(agg.items()):
+            out.append("")
+            out.append(f"| kernel | dispatches | {counter} per dispatch (avg) |")
+            out.append("|---|---|---|")
+            for k, vs in per_kernel.items():
+                out.append(f"| `{k}` | {len(vs)} | {sum(vs) / len(vs):.1f} |")
     return "\n".join(out)
 
 
 def main() -> int:
     if len(sys.argv) > 1 and sys.argv[1] == "--profile-summary":
-        args = sys.argv[2:] + [None, None]
-        print(profile_summary(args[0], args[1], args[2]))
+        print(profile_summary(sys.argv[2], *sys.argv[3:]))
         return 0
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -277,6 +278,8 @@ def main() -> int:
     ap.add_argument("--mixed-qubits", type=int, default=32)
     ap.add_argument("--batch-circuits", type=int, default=2048)
     ap.add_argument("--big-qubits", type=int, default=33, help="0 skips the large-state leg")
+    ap.add_argument("--leg", choices=("qft", "c64"), default=None,
+                    help="run only this leg (for profiling it) and print its figures")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -285,7 +288,18 @@ def main() -> int:
 
     from qiskit_aer_amd import circuits
     n, seed = args.qubits, 2024
+    if args.leg == "qft":
+        q = circuits.qft(n)
+        res = run_engine(n, "double", q, args.steps, args.warmup,
+                         init=lambda qv: qv.initialize_random(seed))
+        res["gates"] = q.count_gates()
+        print(json.dumps({"leg": f"qft_{n}q_c128", **res}), flush=True)
+        return 0
     circ = circuits.brickwork_u3_cx(n, args.layers, seed, measure=False)
+    if args.leg == "c64":
+        res = run_engine(n, "single", circ, args.steps, args.warmup)
+        print(json.dumps({"leg": f"brickwork_{n}q_depth{args.layers}_c64", **res}), flush=True)
+        return 0
     log(f"bench: brickwork {n}q depth {args.layers} c128, {circ.count_gates()} gates")
     c128 = run_engine(n, "double", circ, args.steps, args.warmup)
     # FP64 work of the step: each U3 costs 8 fused multiply-adds per amplitude
