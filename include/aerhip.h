@@ -150,6 +150,14 @@ int aerhip_qv_stats(aerhip_qv *qv, uint64_t *out4);
  * threshold 0; aerhip_run_circuit passes the run configuration's values. */
 int aerhip_qv_set_fusion(aerhip_qv *qv, int enable, int max_qubit, int threshold);
 
+/* No counterpart in the reference (its GPU executor launches every gate of
+ * every run): a flush whose pass launches have the signature of the previous
+ * flush's (the same circuit again, or the same circuit shape with other
+ * angles) runs them as a HIP graph, captured the second time and replayed
+ * from then on.  Returns how many flushes of this handle were captured and
+ * replayed. */
+int aerhip_qv_graph_stats(aerhip_qv *qv, uint64_t *captures, uint64_t *replays);
+
 /* No counterpart in the reference: number of gate primitives merged into
  * fused ops on this handle so far. */
 int aerhip_qv_fused_count(aerhip_qv *qv, uint64_t *out);

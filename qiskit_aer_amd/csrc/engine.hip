@@ -72,6 +72,7 @@ Engine::~Engine() {
   if (plan_only_) return;
   (void)hipSetDevice(device_);
   if (stream_) (void)hipStreamSynchronize(stream_);
+  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
   if (sv_) (void)hipFree(sv_);
   if (d_prog_) (void)hipFree(d_prog_);
   if (d_partial_) (void)hipFree(d_partial_);
@@ -1762,6 +1763,15 @@ void launch_pass(int prec, uint32_t reg_bits, uint32_t n, uint32_t tile_bits, co
   AERHIP_CHECK(hipGetLastError());
 }
 
+// AERHIP_GRAPHS=0 launches every flush's passes one by one (no HIP graphs).
+bool graphs_enabled() {
+  static const bool on = [] {
+    const char *e = std::getenv("AERHIP_GRAPHS");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
 // The register-pass variant of a pass: lean if every op is a lean kind, wide
 // if it also has at most kWidePassOps ops.
 void pass_variant(const PassDesc &pd, const std::vector<ROp<double>> &rops, bool &lean, bool &wide) {
@@ -1838,13 +1848,56 @@ void Engine::flush() {
   }
   if (event_timing_) AERHIP_CHECK(hipEventRecord(t0_ev_, stream_));
   const DeviceProgram dp{dpass, dops, drops, dpool, dprobes, 0};
+  // The launch sequence is a function of this signature: the kernel arguments
+  // (device addresses, offsets into the uploaded program) and each pass's
+  // kernel and grid.  A flush with the signature of the previous one (the same
+  // circuit again, or the same shape with other angles: the new program was
+  // uploaded to the same buffer above) replays the sequence as a HIP graph:
+  // captured on the second occurrence, replayed from the third on.
+  std::vector<uint64_t> sig = {reinterpret_cast<uint64_t>(d_prog_), reinterpret_cast<uint64_t>(sv_),
+                               reinterpret_cast<uint64_t>(dprobes), static_cast<uint64_t>(prec_),
+                               reg_bits_, n_, tile_bits_, passes.size(), ops_off, rops_off, pool_off};
+  std::vector<std::pair<bool, bool>> variant(passes.size());
   for (size_t i = 0; i < passes.size(); ++i) {
     bool lean = false, wide = false;
     pass_variant(passes[i], rops, lean, wide);
-    launch_pass(prec_, reg_bits_, n_, tile_bits_, passes[i], static_cast<uint32_t>(i), lean, wide, dp, sv_, 1, 0,
-                stream_);
-    stats_.launches++;
+    variant[i] = {lean, wide};
+    const PassDesc &pd = passes[i];
+    sig.push_back((uint64_t(pd.kind) << 48) | (uint64_t(pd.ncomp) << 32) | (uint64_t(pd.nb) << 16) |
+                  (uint64_t(lean) << 1) | uint64_t(wide));
   }
+  auto launch_all = [&] {
+    for (size_t i = 0; i < passes.size(); ++i)
+      launch_pass(prec_, reg_bits_, n_, tile_bits_, passes[i], static_cast<uint32_t>(i), variant[i].first,
+                  variant[i].second, dp, sv_, 1, 0, stream_);
+  };
+  if (graphs_enabled() && graph_exec_ && sig == graph_sig_) {
+    AERHIP_CHECK(hipGraphLaunch(graph_exec_, stream_));
+    stats_.graph_replays++;
+  } else if (graphs_enabled() && sig == last_sig_) {
+    if (graph_exec_) AERHIP_CHECK(hipGraphExecDestroy(graph_exec_));
+    graph_exec_ = nullptr;
+    hipGraph_t g = nullptr;
+    AERHIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    try {
+      launch_all();
+    } catch (...) {
+      (void)hipStreamEndCapture(stream_, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    AERHIP_CHECK(hipStreamEndCapture(stream_, &g));
+    const hipError_t ie = hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    AERHIP_CHECK(ie);
+    graph_sig_ = sig;
+    AERHIP_CHECK(hipGraphLaunch(graph_exec_, stream_));
+    stats_.graph_captures++;
+  } else {
+    launch_all();
+  }
+  last_sig_ = std::move(sig);
+  stats_.launches += passes.size();
   stats_.passes += passes.size();
   if (nprobes) {
     // Sum each probe's tile partials in tile order and fetch the values.
@@ -2938,6 +2991,14 @@ int aerhip_qv_stats(aerhip_qv *qv, uint64_t *out3) {  // 4 values
     out3[1] = E(qv).stats().launches;
     out3[2] = E(qv).stats().gates;
     out3[3] = E(qv).stats().gathers;
+  });
+}
+
+int aerhip_qv_graph_stats(aerhip_qv *qv, uint64_t *captures, uint64_t *replays) {
+  return guard([&] {
+    if (!captures || !replays) throw std::invalid_argument("aerhip: null argument");
+    *captures = E(qv).stats().graph_captures;
+    *replays = E(qv).stats().graph_replays;
   });
 }
 

@@ -53,6 +53,8 @@ struct EngineStats {
   uint64_t timed_launches = 0; // launches covered by pass_ms
   uint64_t gathers = 0;        // register layout changes in tile passes
   uint64_t fused = 0;          // gate primitives merged into fused diagonals
+  uint64_t graph_captures = 0; // flushes whose pass launches were captured into a HIP graph
+  uint64_t graph_replays = 0;  // flushes that replayed the captured graph
 };
 
 class Engine {
@@ -202,6 +204,10 @@ class Engine {
   size_t probes_cap_ = 0;
 
   EngineStats stats_;
+  // HIP graph of the last repeated flush (its launch signature and the
+  // instantiated graph) and the signature of the previous flush.
+  std::vector<uint64_t> graph_sig_, last_sig_;
+  hipGraphExec_t graph_exec_ = nullptr;
   bool event_timing_ = false;
   hipEvent_t t0_ev_ = nullptr, t1_ev_ = nullptr;
 };

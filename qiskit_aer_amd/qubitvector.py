@@ -131,6 +131,12 @@ class QubitVector:
         fusion_max_qubit / fusion_threshold)."""
         L.check(self._L.aerhip_qv_set_fusion(self._h, int(bool(enable)), int(max_qubit), int(threshold)))
 
+    def graph_stats(self) -> tuple:
+        """(flushes captured into a HIP graph, flushes that replayed it)."""
+        c, r = C.c_uint64(), C.c_uint64()
+        L.check(self._L.aerhip_qv_graph_stats(self._h, C.byref(c), C.byref(r)))
+        return int(c.value), int(r.value)
+
     def fused_count(self) -> int:
         out = C.c_uint64()
         L.check(self._L.aerhip_qv_fused_count(self._h, C.byref(out)))
