@@ -587,6 +587,33 @@ PYBIND11_MODULE(controller_wrappers, m) {
 #undef AERHIP_FIELD
   cfg.def_property_readonly_static("GLOBAL_PHASE_POS", [](py::object) { return -1; });
 
+  // Classical-expression types of the reference's module
+  // (aer_circuit_binding.hpp:40-130): the compiler imports them, and builds
+  // them only for circuits with classical expressions, whose conditional
+  // operations this engine reports as unsupported.  They take any arguments.
+  enum class UnaryOp { BitNot, LogicNot };
+  py::enum_<UnaryOp>(m, "AerUnaryOp").value("BitNot", UnaryOp::BitNot).value("LogicNot", UnaryOp::LogicNot);
+  enum class BinaryOp { BitAnd, BitOr, BitXor, LogicAnd, LogicOr, Equal, NotEqual, Less, LessEqual, Greater,
+                        GreaterEqual };
+  py::enum_<BinaryOp>(m, "AerBinaryOp")
+      .value("BitAnd", BinaryOp::BitAnd).value("BitOr", BinaryOp::BitOr).value("BitXor", BinaryOp::BitXor)
+      .value("LogicAnd", BinaryOp::LogicAnd).value("LogicOr", BinaryOp::LogicOr)
+      .value("Equal", BinaryOp::Equal).value("NotEqual", BinaryOp::NotEqual).value("Less", BinaryOp::Less)
+      .value("LessEqual", BinaryOp::LessEqual).value("Greater", BinaryOp::Greater)
+      .value("GreaterEqual", BinaryOp::GreaterEqual);
+  struct ExprStub {};
+  // one Python type per name, each a subclass of one registered base (a C++
+  // type is registered once)
+  py::class_<ExprStub>(m, "_AerExprBase", py::dynamic_attr())
+      .def(py::init([](py::args, py::kwargs) { return ExprStub{}; }));
+  {
+    py::object base = m.attr("_AerExprBase");
+    py::object type = py::module_::import("builtins").attr("type");
+    for (const char *name : {"AerScalarType", "AerUint", "AerBool", "AerExpr", "AerCast", "AerVar", "AerValue",
+                             "AerUintValue", "AerBoolValue", "AerUnaryExpr", "AerBinaryExpr"})
+      m.attr(name) = type(name, py::make_tuple(base), py::dict());
+  }
+
   py::class_<AerCircuit, std::shared_ptr<AerCircuit>> circ(m, "AerCircuit");
   circ.def(py::init<>());
   circ.def("__repr__", [](const AerCircuit &c) {

@@ -66,6 +66,21 @@ def test_config_options_as_the_compiler_probes_them():
     assert cw.AerConfig.GLOBAL_PHASE_POS == -1
 
 
+def test_compiler_imports_resolve():
+    # the names aer_compiler.py:47-60 imports from the module
+    for name in ("AerUnaryExpr", "AerUnaryOp", "AerBinaryExpr", "AerBinaryOp", "AerUintValue",
+                 "AerBoolValue", "AerUint", "AerBool", "AerCast", "AerVar", "AerCircuit", "AerConfig",
+                 "aer_controller_execute", "aer_initialize_libraries"):
+        assert hasattr(cw, name), name
+    assert cw.AerBinaryOp.Equal != cw.AerBinaryOp.Less
+    e = cw.AerBinaryExpr(cw.AerBinaryOp.Equal, cw.AerVar(cw.AerUint(2), [0, 1]), cw.AerUintValue(cw.AerUint(2), 1))
+    a = cw.AerCircuit()
+    a.num_qubits = 1
+    a.gate("x", [0], [], [], -1, e, "")
+    r = cw.aer_controller_execute().execute([a], {}, config(1))
+    assert "conditional" in r["results"][0]["status"]
+
+
 def test_circuit_builder_and_unsupported_instructions_fail_loudly():
     a = cw.AerCircuit()
     a.num_qubits = 2
