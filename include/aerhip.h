@@ -158,6 +158,17 @@ int aerhip_qv_set_fusion(aerhip_qv *qv, int enable, int max_qubit, int threshold
  * replayed. */
 int aerhip_qv_graph_stats(aerhip_qv *qv, uint64_t *captures, uint64_t *replays);
 
+/* Block fusion, the engine's form of Aer's fusion of gates into dense
+ * matrices (src/transpile/fusion.hpp:849-1136, fusion_max_qubit): unitary
+ * gates whose qubits stay within `width` qubits (2..5; 0 turns it off, the
+ * default) are multiplied into one matrix, found by looking ahead past gates
+ * on other qubits; blocks on 3 to 5 qubits run as GEMMs on MFMA on the tile
+ * in LDS.  With cost_check = 1 a block is fused only when the engine's cost
+ * model predicts it is cheaper than its gates (Aer weighs its CPU costs the
+ * same way, fusion.hpp:1105-1136); 0 fuses every block of two or more gates.
+ * The autotuner (qiskit_aer_amd.autotune) picks the width by timing. */
+int aerhip_qv_set_block_fusion(aerhip_qv *qv, int width, int cost_check);
+
 /* No counterpart in the reference: number of gate primitives merged into
  * fused ops on this handle so far. */
 int aerhip_qv_fused_count(aerhip_qv *qv, uint64_t *out);

@@ -110,6 +110,7 @@ _SIGNATURES = {
     "aerhip_qv_set_fusion": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "aerhip_qv_fused_count": (C.c_int, [C.c_void_p, _u64p]),
     "aerhip_qv_graph_stats": (C.c_int, [C.c_void_p, _u64p, _u64p]),
+    "aerhip_qv_set_block_fusion": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "aerhip_norm_matrix": (C.c_int, [C.c_void_p, _u64p, C.c_uint64, _dp, _dp]),
     "aerhip_norm_diagonal": (C.c_int, [C.c_void_p, _u64p, C.c_uint64, _dp, _dp]),
     "aerhip_qv_get_state": (C.c_int, [C.c_void_p, C.c_uint64, _dp]),

@@ -1524,7 +1524,224 @@ std::vector<PassPlan> Engine::plan() const {
 // primitives of register passes), ROps (register passes) and the coefficient
 // pool, appended to `prog` (pool offsets are relative to prog.pool).  Consumes
 // the queue.
+// ---------------------------------------------------------------------------
+// Engine-side block fusion: runs of unitary gates whose qubits (targets and
+// controls) stay within `width` qubits are multiplied into one dense matrix
+// (FP64, on the host), which the tile passes apply as a GEMM on MFMA (k = 3..5
+// qubits) or as a register 4x4 / 2x2 gate.  This is the aggregation of Aer's
+// fusion (src/transpile/fusion.hpp:849-966; the fused matrix is the product
+// of the members' matrices, as UnitaryFusion builds it, :159-174), with a
+// cost model of this engine instead of Aer's CPU one (:1105-1136): a block is
+// fused only when its dense form costs less than its members.
+// ---------------------------------------------------------------------------
+namespace {
+
+// Qubits a primitive acts on (targets and controls) and whether block
+// fusion can take it.
+bool block_fusable(const Prim &p, uint32_t width) {
+  if (!p.unitary || p.solo) return false;
+  switch (p.kind) {
+    case K_DENSE1: case K_DENSE: case K_DIAG: case K_MCX: case K_MCY: case K_MCSWAP: case K_MCPHASE:
+      break;
+    default:
+      return false;
+  }
+  return static_cast<uint32_t>(p.q.size() + p.ctrl.size()) <= width;
+}
+
+uint64_t prim_qubits(const Prim &p) {
+  uint64_t m = 0;
+  for (uint32_t q : p.q) m |= 1ull << q;
+  for (uint32_t q : p.ctrl) m |= 1ull << q;
+  return m;
+}
+
+// Applies primitive p to the 2^w-amplitude vector v over the block qubits
+// `bq` (bit i of an index = qubit bq[i]).
+void apply_small(const Prim &p, const std::vector<uint32_t> &bq, const std::vector<cd> &pool, cd *v) {
+  const uint32_t w = static_cast<uint32_t>(bq.size());
+  auto loc = [&](uint32_t q) {
+    for (uint32_t i = 0; i < w; ++i)
+      if (bq[i] == q) return i;
+    throw std::logic_error("aerhip: block fusion: qubit outside the block");
+  };
+  uint32_t cm = 0, cv = 0;  // control mask / required value (local bits)
+  for (size_t i = 0; i < p.ctrl.size(); ++i) {
+    const uint32_t b = 1u << loc(p.ctrl[i]);
+    cm |= b;
+    if ((p.ctrl_val >> i) & 1ull) cv |= b;
+  }
+  const uint32_t S = 1u << w;
+  switch (p.kind) {
+    case K_MCPHASE: {
+      const cd ph(p.pr, p.pi);
+      for (uint32_t x = 0; x < S; ++x)
+        if ((x & cm) == cm) v[x] *= ph;
+      return;
+    }
+    case K_MCX: case K_MCY: {
+      const uint32_t t = 1u << loc(p.q[0]);
+      for (uint32_t x = 0; x < S; ++x) {
+        if ((x & t) || (x & cm) != cv) continue;
+        const cd a0 = v[x], a1 = v[x | t];
+        if (p.kind == K_MCX) {
+          v[x] = a1;
+          v[x | t] = a0;
+        } else {
+          v[x] = cd(0, -1) * a1;
+          v[x | t] = cd(0, 1) * a0;
+        }
+      }
+      return;
+    }
+    case K_MCSWAP: {
+      const uint32_t a = 1u << loc(p.q[0]), b = 1u << loc(p.q[1]);
+      for (uint32_t x = 0; x < S; ++x)
+        if ((x & a) && !(x & b) && (x & cm) == cv) std::swap(v[x], v[(x & ~a) | b]);
+      return;
+    }
+    case K_DIAG: {
+      const uint32_t k = static_cast<uint32_t>(p.q.size());
+      std::vector<uint32_t> tb(k);
+      for (uint32_t j = 0; j < k; ++j) tb[j] = loc(p.q[j]);
+      for (uint32_t x = 0; x < S; ++x) {
+        if ((x & cm) != cv) continue;
+        uint32_t i = 0;
+        for (uint32_t j = 0; j < k; ++j) i |= ((x >> tb[j]) & 1u) << j;
+        v[x] *= pool[p.off + i];
+      }
+      return;
+    }
+    default: {  // K_DENSE1 / K_DENSE: column-major matrix, bit j of its index = q[j]
+      const uint32_t k = static_cast<uint32_t>(p.q.size()), D = 1u << k;
+      uint32_t tm = 0;
+      std::vector<uint32_t> tb(k);
+      for (uint32_t j = 0; j < k; ++j) {
+        tb[j] = loc(p.q[j]);
+        tm |= 1u << tb[j];
+      }
+      std::vector<cd> in(D);
+      for (uint32_t x = 0; x < S; ++x) {
+        if ((x & tm) || (x & cm) != cv) continue;
+        auto at = [&](uint32_t i) {
+          uint32_t y = x;
+          for (uint32_t j = 0; j < k; ++j) y |= ((i >> j) & 1u) << tb[j];
+          return y;
+        };
+        for (uint32_t i = 0; i < D; ++i) in[i] = v[at(i)];
+        for (uint32_t r = 0; r < D; ++r) {
+          cd acc = 0;
+          for (uint32_t c = 0; c < D; ++c) acc += pool[p.off + r + D * c] * in[c];
+          v[at(r)] = acc;
+        }
+      }
+      return;
+    }
+  }
+}
+
+// Relative cost per amplitude of running a primitive (or a fused block) in a
+// tile pass: arithmetic in units of one FP64 FMA (complex64 FMAs on MFMA cost
+// half), plus the fixed cost of an op and of moving the tile to LDS for a
+// dense matrix on 3 or more qubits.  Measured op times on 28 qubits (c128 /
+// c64 us): U3 60 / 42, 4x4 167 / 108, CZ 77 / 51, layout change 95 / 62.
+double block_cost(uint32_t kind, uint32_t k, bool single) {
+  const double op = 4.0;                      // dispatch and register bookkeeping
+  switch (kind) {
+    case K_MCX: case K_MCY: case K_MCSWAP: return 1.0;
+    case K_MCPHASE: case K_DIAG: return op + 4.0;
+    case K_DENSE1: return op + (single ? 4.0 : 8.0);
+    default: break;
+  }
+  const double fma = 4.0 * static_cast<double>(1u << k);  // complex MACs x 4 FMAs per amplitude
+  if (k <= 2) return op + fma;                            // register 4x4 on the vector ALU
+  return op + 12.0 + (single ? fma / 2.0 : fma);          // LDS round trip + MFMA GEMM
+}
+
+}  // namespace
+
+std::vector<Prim> Engine::fuse_blocks(const std::vector<Prim> &in) {
+  const uint32_t width = block_width_;
+  const bool single = prec_ == AERHIP_PRECISION_SINGLE;
+  const size_t window = 128;
+  std::vector<char> used(in.size(), 0);
+  std::vector<Prim> out;
+  out.reserve(in.size());
+  for (size_t i = 0; i < in.size(); ++i) {
+    if (used[i]) continue;
+    const Prim &p0 = in[i];
+    if (!block_fusable(p0, width)) {
+      out.push_back(p0);
+      continue;
+    }
+    // Members: p0 and later fusable primitives on the block's qubits (at most
+    // `width`) that commute with every primitive they are moved past (those
+    // share no qubit with them).
+    uint64_t Q = prim_qubits(p0), blocked = 0;
+    std::vector<size_t> members = {i};
+    for (size_t j = i + 1; j < in.size() && j < i + window; ++j) {
+      if (used[j]) continue;
+      const Prim &p = in[j];
+      const uint64_t t = p.touch;
+      if (!block_fusable(p, width)) {
+        blocked |= t;
+      } else {
+        const uint64_t qp = prim_qubits(p);
+        if (!(qp & blocked) && static_cast<uint32_t>(__builtin_popcountll(Q | qp)) <= width) {
+          members.push_back(j);
+          Q |= qp;
+        } else {
+          blocked |= qp;
+        }
+      }
+      if ((blocked & Q) == Q) break;
+    }
+    if (members.size() < 2) {
+      out.push_back(p0);
+      continue;
+    }
+    std::vector<uint32_t> bq;
+    for (uint32_t q = 0; q < 64; ++q)
+      if (Q & (1ull << q)) bq.push_back(q);
+    const uint32_t w = static_cast<uint32_t>(bq.size());
+    double member_cost = 0;
+    for (size_t m : members) member_cost += block_cost(in[m].kind, static_cast<uint32_t>(in[m].q.size()), single);
+    if (block_cost_check_ && block_cost(K_DENSE, w, single) >= member_cost) {
+      out.push_back(p0);
+      continue;
+    }
+    // the block's matrix: column c is the image of basis state c
+    const uint32_t D = 1u << w;
+    std::vector<cd> M(static_cast<size_t>(D) * D);
+    for (uint32_t c = 0; c < D; ++c) {
+      cd *col = M.data() + static_cast<size_t>(D) * c;
+      col[c] = 1.0;
+      for (size_t m : members) apply_small(in[m], bq, pool_, col);
+    }
+    for (size_t m : members) used[m] = 1;
+    Prim f;
+    f.kind = w == 1 ? K_DENSE1 : K_DENSE;
+    f.q = bq;
+    f.len = D * D;
+    f.off = pool_add(M.data(), f.len);
+    f.tmask = Q;
+    f.touch = Q;
+    f.unitary = true;
+    out.push_back(std::move(f));
+    stats_.fused += members.size();
+  }
+  return out;
+}
+
+void Engine::set_block_fusion(uint32_t width, bool cost_check) {
+  if (width == 1 || width > 5) throw std::invalid_argument("aerhip: block fusion width must be 0 or 2..5");
+  flush();
+  block_width_ = width;
+  block_cost_check_ = cost_check;
+}
+
 void Engine::emit_program(Program &prog) {
+  if (block_width_ >= 2 && n_ > fusion_threshold_) queue_ = fuse_blocks(queue_);
   const std::vector<PassPlan> plans = plan();
   std::vector<PassDesc> &passes = prog.passes;
   std::vector<TileOp> &ops = prog.ops;
@@ -2991,6 +3208,13 @@ int aerhip_qv_stats(aerhip_qv *qv, uint64_t *out3) {  // 4 values
     out3[1] = E(qv).stats().launches;
     out3[2] = E(qv).stats().gates;
     out3[3] = E(qv).stats().gathers;
+  });
+}
+
+int aerhip_qv_set_block_fusion(aerhip_qv *qv, int width, int cost_check) {
+  return guard([&] {
+    if (width < 0) throw std::invalid_argument("aerhip: block fusion width must be 0 or 2..5");
+    E(qv).set_block_fusion(static_cast<uint32_t>(width), cost_check != 0);
   });
 }
 

@@ -92,6 +92,10 @@ class Engine {
   // Engine-side fusion of diagonal runs (Aer's fusion options): on/off, the
   // most qubits of one fused op, and the qubit count above which it applies.
   void set_fusion(bool enable, uint32_t max_qubit, uint32_t threshold);
+  // Block fusion of unitary gates into dense matrices on at most `width`
+  // qubits (0: off); with cost_check, a block is fused only when the cost
+  // model predicts a gain.
+  void set_block_fusion(uint32_t width, bool cost_check);
 
   // gates (queued)
   void apply_matrix(const std::vector<uint32_t> &qubits, const std::vector<cd> &mat);
@@ -158,6 +162,7 @@ class Engine {
   void apply_matrix_1(uint32_t q, const std::vector<cd> &mat);
   void apply_diagonal_1(uint32_t q, const cd *diag);
   std::vector<PassPlan> plan() const;
+  std::vector<Prim> fuse_blocks(const std::vector<Prim> &in);
   void ensure_partial(size_t n);
   void *scratch(size_t bytes);
   double fetch_scalar(const double *dptr);
@@ -181,6 +186,8 @@ class Engine {
   bool fusion_enable_ = true;
   uint32_t fusion_max_qubit_ = 64;
   uint32_t fusion_threshold_ = 0;
+  uint32_t block_width_ = 0;     // block fusion width (0: off)
+  bool block_cost_check_ = true;
 
   std::vector<Prim> queue_;
   std::vector<cd> pool_;

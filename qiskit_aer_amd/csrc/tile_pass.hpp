@@ -66,6 +66,85 @@ __device__ __forceinline__ double block_sum_nt(double v, double *red) {
   return s;
 }
 
+// Dense matrix M on k = 3..5 tile qubits without local controls, applied to
+// the LDS tile as a GEMM on MFMA: with D = 2^k and C = N / D columns (the
+// other tile bits), Out = R V, R the 2D x 2D real form of M and V the 2D x C
+// real form of the tile (rows re / im of each amplitude of a column).  R V is
+// cut into 16 x 16 output blocks, each a chain of 2D / 4 MFMA issues with
+// K = 4 (v_mfma_f32_16x16x4f32 for complex64, v_mfma_f64_16x16x4f64 for
+// complex128): lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16].
+// The real K index 4 kk + l / 16 is (column 2 kk + (l / 16 >> 1) of M,
+// part l / 16 & 1), and the real rows are ordered so that each lane's four
+// outputs are the (re, im) of two amplitudes of its column: complex64 blocks
+// leave D[4 (l / 16) + r][l % 16] in lane l, complex128 blocks
+// D[l / 16 + 4 r][l % 16].  `doff[c]` (c < D) holds the tile offset of row c.
+// Returns false (nothing done) when the tile is too small for 16-wide blocks.
+template <typename T, int NT, int BMAX>
+__device__ __forceinline__ bool lds_gemm(const TileOp &op, cx<T> *tile, uint32_t N,
+                                         const cx<double> *__restrict__ pool, const uint32_t *doff) {
+  constexpr uint32_t NW = NT / 64;
+  constexpr int RES = ((1 << BMAX) / 128 / NW) > 0 ? (1 << BMAX) / 128 / NW : 1;
+  const uint32_t k = op.k, D = 1u << k, Dr = 2u * D;
+  const uint32_t C = N >> k;
+  if (k < 3 || k > 5 || C < 16 || op.cl != 0) return false;
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t kq = lane >> 4, l16 = lane & 15u;
+  uint32_t tmask = 0;
+  for (uint32_t b = 0; b < k; ++b) tmask |= 1u << op.pos[b];
+  const uint32_t row_tiles = Dr / 16u, ntiles = row_tiles * (C / 16u), ksteps = Dr / 4u;
+  // this lane's row of A (complex row within the block and part)
+  uint32_t arow, apart;
+  if constexpr (sizeof(T) == 4) {
+    arow = l16 >> 1;
+    apart = l16 & 1u;
+  } else {
+    arow = 2u * (l16 & 3u) + ((l16 >> 2) >> 1);
+    apart = (l16 >> 2) & 1u;
+  }
+  cx<T> res[RES][2];
+  uint32_t coffs[RES];
+  uint32_t s = 0;
+  for (uint32_t t = wave; t < ntiles; t += NW, ++s) {
+    const uint32_t ti = t % row_tiles, tj = t / row_tiles;
+    // tile offset of column j = 16 tj + l16: its bits deposited on the
+    // non-target tile bits, ascending
+    uint32_t j = 16u * tj + l16, coff = 0;
+    for (uint32_t b = 0; j && (1u << b) < N; ++b)
+      if (!(tmask & (1u << b))) {
+        coff |= (j & 1u) << b;
+        j >>= 1;
+      }
+    const uint32_t c = 8u * ti + arow;
+    typedef T v4 __attribute__((ext_vector_type(4)));
+    v4 acc = {T(0), T(0), T(0), T(0)};
+    for (uint32_t kk = 0; kk < ksteps; ++kk) {
+      const uint32_t cc = 2u * kk + (kq >> 1), q = kq & 1u;
+      const cx<double> m = pool[op.off + c + D * cc];
+      const double av = apart ? (q ? m.x : m.y) : (q ? -m.y : m.x);
+      const cx<T> v = tile[doff[cc] | coff];
+      const T bv = q ? v.y : v.x;
+      if constexpr (sizeof(T) == 4)
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(static_cast<float>(av), bv, acc, 0, 0, 0);
+      else
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+    }
+    if (s < static_cast<uint32_t>(RES)) {
+      res[s][0] = mk<T>(acc[0], acc[1]);
+      res[s][1] = mk<T>(acc[2], acc[3]);
+      coffs[s] = coff | (ti << 16);
+    }
+  }
+  __syncthreads();  // every read of the tile is done before the writes
+  s = 0;
+  for (uint32_t t = wave; t < ntiles && s < static_cast<uint32_t>(RES); t += NW, ++s) {
+    const uint32_t coff = coffs[s] & 0xFFFFu, ti = coffs[s] >> 16;
+    const uint32_t c0 = 8u * ti + 2u * kq;  // the lane's two output rows
+    tile[doff[c0] | coff] = res[s][0];
+    tile[doff[c0 + 1] | coff] = res[s][1];
+  }
+  return true;
+}
+
 // LDS-resident primitive (the tile is in `tile`); every thread takes part.
 // K_PROBE hands the tile's share of a Pauli expectation value to `sink`.
 // MATCAP: complex entries of `mat`; larger matrices are read from the pool.
@@ -127,6 +206,17 @@ __device__ __forceinline__ void lds_apply(const TileOp &op, cx<T> *tile, cx<T> *
       const uint32_t k = op.k, D = 1u << k;
       uint32_t tmask = 0;
       for (uint32_t b = 0; b < k; ++b) tmask |= 1u << op.pos[b];
+      if (k >= 3 && k <= 5 && cl == 0 && (N >> k) >= 16) {
+        // matrices on 3 to 5 tile qubits: a GEMM on MFMA
+        if (tid < D) {
+          uint32_t d = 0;
+          for (uint32_t b = 0; b < k; ++b) d |= ((tid >> b) & 1u) << op.pos[b];
+          doff[tid] = d;
+        }
+        __syncthreads();
+        lds_gemm<T, NT, BMAX>(op, tile, N, pool, doff);
+        break;
+      }
       const bool staged = D * D <= static_cast<uint32_t>(MATCAP);
       if (staged)
         for (uint32_t e = tid; e < D * D; e += NT) mat[e] = to_t<T>(pool[op.off + e]);

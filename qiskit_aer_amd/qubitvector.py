@@ -131,6 +131,11 @@ class QubitVector:
         fusion_max_qubit / fusion_threshold)."""
         L.check(self._L.aerhip_qv_set_fusion(self._h, int(bool(enable)), int(max_qubit), int(threshold)))
 
+    def set_block_fusion(self, width: int, cost_check: bool = True) -> None:
+        """Fuses runs of unitary gates on at most `width` qubits (2..5, 0: off)
+        into dense matrices (GEMMs on MFMA for 3..5 qubits)."""
+        L.check(self._L.aerhip_qv_set_block_fusion(self._h, int(width), int(bool(cost_check))))
+
     def graph_stats(self) -> tuple:
         """(flushes captured into a HIP graph, flushes that replayed it)."""
         c, r = C.c_uint64(), C.c_uint64()
