@@ -139,9 +139,20 @@ def run_mixed_precision(n: int, layers: int, seed: int) -> dict:
     states resident on the device: gate throughput of each, 1 - |<psi64|psi128>|^2
     and the mean |error| of <Z_i Z_{i+1}>."""
     from qiskit_aer_amd import QubitVector, circuits
+    from qiskit_aer_amd.autotune import tune
     c = circuits.su4_brickwork(n, layers, seed)
     packed = c.pack()
     out = {"gates": c.count_gates(), "num_qubits": n, "layers": layers}
+    # Block-fusion width per precision, picked by the autotuner on the same
+    # brickwork at 26 qubits (the 32-qubit runs below use the chosen width).
+    proxy = circuits.su4_brickwork(min(n, 26), layers, seed)
+    width = {}
+    out["autotune"] = {}
+    for prec in ("double", "single"):
+        best, times = tune(proxy, prec, widths=(0, 4, 5))
+        width[prec] = best
+        out["autotune"][prec] = {"proxy_qubits": proxy.num_qubits, "best_width": best,
+                                 "ms_by_width": times}
     states = {}
     labels = []
     for i in range(n - 1):
@@ -151,6 +162,7 @@ def run_mixed_precision(n: int, layers: int, seed: int) -> dict:
     zz = {}
     for prec in ("double", "single"):
         qv = QubitVector(n, prec)
+        qv.set_block_fusion(width[prec])
         qv.initialize()
         qv.synchronize()
         qv.set_event_timing(True)
