@@ -290,8 +290,12 @@ def main() -> int:
     ap.add_argument("--mixed-qubits", type=int, default=32)
     ap.add_argument("--batch-circuits", type=int, default=2048)
     ap.add_argument("--big-qubits", type=int, default=33, help="0 skips the large-state leg")
-    ap.add_argument("--leg", choices=("qft", "c64"), default=None,
+    ap.add_argument("--leg", choices=("qft", "c64", "su4"), default=None,
                     help="run only this leg (for profiling it) and print its figures")
+    ap.add_argument("--precision", choices=("double", "single"), default="double",
+                    help="precision of the su4 leg")
+    ap.add_argument("--fusion-width", type=int, default=0,
+                    help="block-fusion width of the su4 leg (0: off, 2..5)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -306,6 +310,25 @@ def main() -> int:
                          init=lambda qv: qv.initialize_random(seed))
         res["gates"] = q.count_gates()
         print(json.dumps({"leg": f"qft_{n}q_c128", **res}), flush=True)
+        return 0
+    if args.leg == "su4":
+        from qiskit_aer_amd import QubitVector
+        nq = args.mixed_qubits
+        c = circuits.su4_brickwork(nq, 20, seed)
+        qv = QubitVector(nq, args.precision)
+        qv.set_block_fusion(args.fusion_width, cost_check=True)
+        res = {}
+        for step in range(args.warmup + args.steps):
+            qv.initialize()
+            qv.synchronize()
+            t0 = time.perf_counter()
+            qv.apply_ops(c)
+            qv.synchronize()
+            res["ms"] = 1e3 * (time.perf_counter() - t0)
+        res.update({"fused_gates": qv.fused_count(), "gates": c.count_gates(), "num_qubits": nq,
+                    "precision": args.precision, "fusion_width": args.fusion_width})
+        qv.close()
+        print(json.dumps({"leg": "su4_brickwork", **res}), flush=True)
         return 0
     circ = circuits.brickwork_u3_cx(n, args.layers, seed, measure=False)
     if args.leg == "c64":
