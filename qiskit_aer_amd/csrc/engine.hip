@@ -927,6 +927,15 @@ std::vector<TileOp> fuse_phase_runs(const std::vector<TileOp> &in, const uint8_t
 // Walks a register program the way k_reg_pass does and checks its shape and
 // every layout, so that a malformed program fails on the host and never
 // reaches the device.
+// Moves a register op's pool and TileOp references by the offsets of its
+// program inside a concatenated one.
+void rebase_rop(ROp<double> &r, uint32_t pool_base, uint32_t op_base) {
+  const uint32_t kk = r.kind & ~F_ALL;
+  if (kk == Q_U2 || kk == Q_DIAGF) r.extra += pool_base;
+  else if (kk == Q_DIAG || kk == Q_LDS) r.extra += op_base;
+  else if ((kk == Q_LOAD || kk == Q_FILL) && r.a) r.b += pool_base;
+}
+
 void validate_register_program(const ROp<double> *p, size_t n, uint32_t B, uint32_t R,
                                const uint8_t *spos, size_t ntops) {
   auto fail = [](const char *what) { throw std::logic_error(std::string("aerhip: register program: ") + what); };
@@ -964,6 +973,7 @@ void validate_register_program(const ROp<double> *p, size_t n, uint32_t B, uint3
     if ((o == 1) != (e.kind == Q_LOAD)) fail("LOAD must come first, and only there");
     check_layout(e.u.lay);
     if (e.extra & 1u) fail("odd phase length");
+    if (e.a > kDiagSlots) fail("fused-diagonal slots");
     if (o + e.extra >= n) fail("phase overruns the program");
     for (uint32_t i = 0; i < e.extra; ++i) {
       const ROp<double> &r = p[o++];
@@ -987,7 +997,9 @@ void validate_register_program(const ROp<double> *p, size_t n, uint32_t B, uint3
         case Q_DIAG: if (r.extra >= ntops) fail("diagonal index"); break;
         case Q_PROBE: if (r.u.pr.xr >= (1u << R)) fail("probe mask"); break;
         case Q_PHASE: case Q_NOP: break;
-        case Q_DIAGF: if (r.a > 8) fail("fused diagonal tables"); break;
+        case Q_DIAGF:
+          if ((r.a & 0xFFu) > 8 || (r.a & 0xFF00u) || (r.a >> 16) > e.a) fail("fused diagonal tables");
+          break;
         default: fail("op kind inside a register phase");
       }
     }
@@ -1128,6 +1140,37 @@ void register_program(const std::vector<TileOp> &in, uint32_t B, uint32_t R, uin
     r.map_t0 = static_cast<uint16_t>(tile_mask(cinv));
     map_dirty = false;
   };
+  // Fused diagonals of the phase that ends: slot k for the k-th (up to
+  // kDiagSlots), lane record 8k + c for its table c, after an entry equal to 1
+  // (the table of unused lanes); table distances are counted back from the
+  // first record, so that moving the pool moves nothing else.
+  auto assign_diag_slots = [&] {
+    const uint32_t NR = 1u << R;
+    std::vector<ROp<double> *> df;
+    for (size_t i = enter_at + 1; i < rops.size() && df.size() < kDiagSlots; ++i)
+      if ((rops[i].kind & ~F_ALL) == Q_DIAGF) df.push_back(&rops[i]);
+    if (df.empty()) return;
+    pool.push_back(cd(1.0, 0.0));
+    const size_t first = pool.size();
+    if (first + 32 > 0xFFFFFFFFull) throw std::length_error("aerhip: pool offset");
+    DiagLane lanes[64];
+    for (uint32_t l = 0; l < 64; ++l) {
+      lanes[l].back = 1;
+      lanes[l].fq = 0xFFFFFFFFu;
+      const uint32_t k = l / 8, c = l % 8;
+      if (k >= df.size() || c >= (df[k]->a & 0xFFu)) continue;
+      lanes[l].back = static_cast<uint32_t>(first - (df[k]->extra + NR + 32 + 16 * c));
+      lanes[l].fq = 0;
+      for (uint32_t w = 0; w < 4; ++w) lanes[l].fq |= static_cast<uint32_t>(df[k]->u.fq[4 * c + w]) << (8 * w);
+    }
+    cd w[32];
+    std::memcpy(w, lanes, sizeof(lanes));
+    pool.insert(pool.end(), w, w + 32);
+    for (size_t k = 0; k < df.size(); ++k) df[k]->a = (df[k]->a & 0xFFu) | static_cast<uint32_t>(k + 1) << 16;
+    ROp<double> &e = rops[enter_at];
+    e.a = static_cast<uint32_t>(df.size());
+    e.b = static_cast<uint32_t>(first);
+  };
   auto emit_raw = [&](uint32_t kind, uint32_t rs) {
     if (kind == Q_STORE || kind == Q_SPILL) {
       uint32_t cnt = static_cast<uint32_t>(rops.size() - enter_at - 1);
@@ -1139,6 +1182,7 @@ void register_program(const std::vector<TileOp> &in, uint32_t B, uint32_t R, uin
         ++cnt;
       }
       rops[enter_at].extra = cnt;
+      assign_diag_slots();
     }
     if (kind == Q_LOAD || kind == Q_FILL) enter_at = rops.size();
     ROp<double> r;
@@ -2640,11 +2684,7 @@ void run_batch(Engine *const *engines, uint64_t count, int device, double *value
       t.off += pb;
       ho[op_at[i] + k] = t;
     }
-    for (ROp<double> &r : p.rops) {
-      const uint32_t kk = r.kind & ~F_ALL;
-      if (kk == Q_U2 || kk == Q_DIAGF) r.extra += pb;
-      else if (kk == Q_DIAG || kk == Q_LDS) r.extra += ob;
-    }
+    for (ROp<double> &r : p.rops) rebase_rop(r, pb, ob);
     if (prec == AERHIP_PRECISION_DOUBLE)
       std::memcpy(h.get() + rops_off + rop_at[i] * sizeof(ROp<double>), p.rops.data(),
                   p.rops.size() * sizeof(ROp<double>));
@@ -2861,9 +2901,7 @@ class MultiState {
         ops.push_back(t);
       }
       for (ROp<double> r : p.rops) {
-        const uint32_t kk = r.kind & ~F_ALL;
-        if (kk == Q_U2 || kk == Q_DIAGF) r.extra += pb;
-        else if (kk == Q_DIAG || kk == Q_LDS) r.extra += ob;
+        rebase_rop(r, pb, ob);
         rops.push_back(r);
       }
       pool.insert(pool.end(), p.pool.begin(), p.pool.end());

@@ -28,11 +28,13 @@
 namespace aerhip {
 
 enum RegPassOpKind : uint32_t {
-  Q_LOAD = 40,      // HBM -> registers in layout u.lay; extra: register ops that follow
+  Q_LOAD = 40,      // HBM -> registers in layout u.lay; extra: register ops that follow;
+                    // a: fused-diagonal slots of the phase, b: pool offset of their
+                    // DiagLane records
   Q_STORE = 41,     // registers -> HBM in layout u.lay (extra = 1: read-only pass, no write)
   Q_RELAYOUT = 42,  // registers -> LDS -> registers in layout u.lay
   Q_SPILL = 43,     // registers -> LDS (the tile is then in LDS)
-  Q_FILL = 44,      // LDS -> registers in layout u.lay; extra: register ops that follow
+  Q_FILL = 44,      // LDS -> registers in layout u.lay; extra, a, b: as Q_LOAD
   Q_LDS = 45,       // TileOp primitive extra on the LDS tile
   Q_U1 = 46,        // 2x2 matrix u.m on register bit a
   Q_U1C = 47,       // 2x2 matrix on register bit a where (l & cl) == cvl
@@ -53,9 +55,32 @@ enum RegPassOpKind : uint32_t {
   Q_DIAGF = 60,     // fused diagonal: where (l & cl) == cvl (registers: bit j of b), the
                     // product over its member qubits of (bit ? v : u), from host tables at
                     // pool[extra..]: NR register factors, 16 + 16 lane factors (thread bits
-                    // 0-3, 4-7), then a = number of 16-entry tables over fixed qubits
-                    // u.fq[4c .. 4c+3] (0xFF: unused)
+                    // 0-3, 4-7), then a & 0xFF = number of 16-entry tables over fixed
+                    // qubits u.fq[4c .. 4c+3] (0xFF: unused); a >> 16 = slot + 1 of the
+                    // phase's precomputed fixed-qubit factor (0: none, read the tables)
 };
+
+// Fused diagonals of a register phase: the factor over the qubits fixed by
+// the tile is uniform per tile, so it is computed once per phase for up to
+// kDiagSlots ops, while the tile loads are in flight, instead of as a chain of
+// dependent table loads inside each op.  Lane l of a wave reads table l % 8 of
+// slot l / 8, and the eight lanes of a slot multiply their entries together.
+// One 8-byte record per lane in the pool: the distance back from the first
+// record to the table (an entry equal to 1 just before the records for an
+// unused table or slot), and the table's four qubits (0xFF: unused).
+constexpr uint32_t kDiagSlots = 8;
+// Where the phase keeps the slot factors: in a register (lane 8k: slot k)
+// for the lean complex128 variant, whose five-tile LDS budget has no room
+// left; elsewhere in LDS, one row per wave (no register to spare there).
+template <typename T, bool FULL>
+constexpr bool diag_in_lds() {
+  return !(sizeof(T) == 8 && !FULL);
+}
+struct DiagLane {
+  uint32_t back;
+  uint32_t fq;
+};
+static_assert(sizeof(DiagLane) * 64 == 32 * sizeof(cx<double>), "64 records in 32 pool entries");
 // Flag on kind: the op carries a new register relabelling (map_bm, map_t0).
 constexpr uint32_t F_MAP = 0x80000000u;
 // Q_U1X / Q_U1MX: where the tile predicate (base & cg) == cvg holds, an X on
@@ -655,6 +680,23 @@ __device__ __forceinline__ uint64_t uniform64(uint64_t v) {
   return (static_cast<uint64_t>(hi) << 32) | lo;
 }
 
+// Lane l's value of a wave-uniform lane index.
+__device__ __forceinline__ cx<double> read_lane(const cx<double> &v, uint32_t l) {
+  auto rd = [l](double x) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(u)), static_cast<int>(l));
+    const uint32_t hi = __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(u >> 32)), static_cast<int>(l));
+    return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+  };
+  return mk<double>(rd(v.x), rd(v.y));
+}
+__device__ __forceinline__ cx<float> read_lane(const cx<float> &v, uint32_t l) {
+  auto rd = [l](float x) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), static_cast<int>(l)));
+  };
+  return mk<float>(rd(v.x), rd(v.y));
+}
+
 // The value itself, hidden from the optimizer (no code is emitted).
 __device__ __forceinline__ uint32_t opaque_u32(uint32_t v) {
   asm volatile("" : "+s"(v));
@@ -781,7 +823,8 @@ __device__ __forceinline__ Amps<T, 1 << R> rw_op(const ROp<T> &op, const OpHead<
                                                  double *red, uint64_t base, uint32_t tid,
                                                  const TileOp *__restrict__ tops,
                                                  const cx<double> *__restrict__ pool,
-                                                 const ProbeSink &sink) {
+                                                 const ProbeSink &sink, const cx<T> &gdf,
+                                                 const cx<T> *gsh) {
   constexpr int NR = 1 << R;
   constexpr int TB = B - R;
   const uint32_t kind = hd.kind & ~F_ALL, ra = hd.a, rb = hd.b, extra = hd.extra;
@@ -902,17 +945,29 @@ __device__ __forceinline__ Amps<T, 1 << R> rw_op(const ROp<T> &op, const OpHead<
     const uint32_t clt = cl & ~rbits;
     const bool lane_ok = (st.tl & clt) == (cvl & clt);
     const cx<double> *blk = pool + extra;
-    cx<double> g = blk[NR + (tid & 15u)];
-    g = cmul(g, blk[NR + 16 + ((tid >> 4) & 15u)]);
-    const cx<double> *tab = blk + NR + 32;
-    for (uint32_t c = 0; c < ra; ++c, tab += 16) {
-      uint32_t idx = 0;
+    // the lane indices are derived here from an opaque copy of the thread
+    // index: hoisted out of the op loop they would hold registers for the
+    // whole pass
+    uint32_t t = tid;
+    asm volatile("" : "+v"(t));
+    cx<double> g = blk[NR + (t & 15u)];
+    g = cmul(g, blk[NR + 16 + ((t >> 4) & 15u)]);
+    if (const uint32_t slot = ra >> 16) {
+      cx<T> gs;
+      if constexpr (diag_in_lds<T, FULL>()) gs = gsh[(t >> 6) * kDiagSlots + slot - 1u];
+      else gs = read_lane(gdf, 8u * (slot - 1u));
+      g = cmul(g, mk<double>(gs.x, gs.y));
+    } else {
+      const cx<double> *tab = blk + NR + 32;
+      for (uint32_t c = 0; c < (ra & 0xFFu); ++c, tab += 16) {
+        uint32_t idx = 0;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const uint32_t q = op.u.fq[4 * c + i];
-        if (q < 64) idx |= static_cast<uint32_t>((base >> q) & 1ull) << i;
+        for (int i = 0; i < 4; ++i) {
+          const uint32_t q = op.u.fq[4 * c + i];
+          if (q < 64) idx |= static_cast<uint32_t>((base >> q) & 1ull) << i;
+        }
+        g = cmul(g, tab[idx]);
       }
-      g = cmul(g, tab[idx]);
     }
 #pragma unroll
     for (int j = 0; j < NR; ++j) {
@@ -1014,6 +1069,21 @@ __device__ __forceinline__ Amps<T, 1 << R> rw_op(const ROp<T> &op, const OpHead<
   return o;
 }
 
+// The product of the slot's table entries at the tile's fixed-qubit bits,
+// in every lane of the slot's eight.
+__device__ __forceinline__ cx<double> slot_factor(uint2 rec, const cx<double> *dlane, uint64_t base) {
+  uint32_t idx = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const uint32_t q = (rec.y >> (8 * i)) & 0xFFu;
+    if (q < 64) idx |= static_cast<uint32_t>((base >> q) & 1ull) << i;
+  }
+  cx<double> g = *(dlane - static_cast<int64_t>(rec.x) + idx);
+#pragma unroll
+  for (int m = 1; m < 8; m <<= 1) g = cmul(g, mk<double>(__shfl_xor(g.x, m), __shfl_xor(g.y, m)));
+  return g;
+}
+
 // One tile of a register-layout pass: ops[pd->op_begin .. + pd->nops) are
 // ROps; TileOp primitives (Q_LDS, Q_DIAG) index `tops`.  The op list has the
 // form
@@ -1025,7 +1095,7 @@ __device__ __forceinline__ void reg_tile(cx<T> *__restrict__ sv, const PassDesc 
                                          const TileOp *__restrict__ tops,
                                          const cx<double> *__restrict__ pool,
                                          const ProbeSink &sink, uint64_t tile_id, cx<T> *lds,
-                                         cx<T> *mat, uint32_t *doff, double *red) {
+                                         cx<T> *mat, uint32_t *doff, double *red, cx<T> *gsh) {
   constexpr int TB = B - R;
   constexpr int NT = 1 << TB;
   constexpr int NR = 1 << R;
@@ -1046,6 +1116,12 @@ __device__ __forceinline__ void reg_tile(cx<T> *__restrict__ sv, const PassDesc 
     const ROp<T> &e = opl[o++];
     if (e.kind == Q_FILL) __syncthreads();
     LayoutState<NR> st = make_state<R, TB>(e.u.lay, tid);
+    // The lane records are read before the tile, so that their wait does not
+    // wait for the tile loads; the table entries after, overlapping them.
+    const bool has_df = e.a != 0;
+    const cx<double> *dlane = pool + e.b;
+    uint2 drec = make_uint2(0u, 0u);
+    if (has_df) drec = reinterpret_cast<const uint2 *>(dlane)[tid & 63u];
     Amps<T, NR> a;
     if (e.kind == Q_LOAD) {
       const RegOff<NR> ro = reg_offset<R>(e.u.lay);
@@ -1056,6 +1132,16 @@ __device__ __forceinline__ void reg_tile(cx<T> *__restrict__ sv, const PassDesc 
 #pragma unroll
       for (int j = 0; j < NR; ++j) a.v[j] = lds[st.tl | st.ri.at(j)];
     }
+    // held in the amplitude precision (complex64: two registers or 8 bytes)
+    cx<T> gdf = mk<T>(1, 0);
+    if (has_df) {
+      const cx<T> g = to_t<T>(slot_factor(drec, dlane, base));
+      if constexpr (diag_in_lds<T, FULL>()) {
+        if ((tid & 7u) == 0) gsh[(tid >> 6) * kDiagSlots + ((tid & 63u) >> 3)] = g;
+      } else {
+        gdf = g;
+      }
+    }
     // Register ops alternate between banks a and b.  The head of the op
     // after the current one is loaded while the current op runs.
     const uint32_t cnt = e.extra;
@@ -1063,9 +1149,10 @@ __device__ __forceinline__ void reg_tile(cx<T> *__restrict__ sv, const PassDesc 
     OpHead<T, !FULL> h0 = load_head<!FULL>(opp[0]);
     for (uint32_t i = 0; i < cnt; i += 2) {
       const OpHead<T, !FULL> h1 = load_head<!FULL>(opp[1]);
-      const Amps<T, NR> b = rw_op<T, B, R, FULL>(opp[0], h0, a, st, lds, red, base, tid, tops, pool, sink);
+      const Amps<T, NR> b =
+          rw_op<T, B, R, FULL>(opp[0], h0, a, st, lds, red, base, tid, tops, pool, sink, gdf, gsh);
       h0 = load_head<!FULL>(opp[2]);
-      a = rw_op<T, B, R, FULL>(opp[1], h1, b, st, lds, red, base, tid, tops, pool, sink);
+      a = rw_op<T, B, R, FULL>(opp[1], h1, b, st, lds, red, base, tid, tops, pool, sink, gdf, gsh);
       opp += 2;
     }
     o += cnt;
@@ -1124,10 +1211,11 @@ __attribute__((amdgpu_waves_per_eu(reg_pass_waves<T, B, R, WIDE>()))) void k_reg
   __shared__ cx<T> mat[16];
   __shared__ uint32_t doff[32];
   __shared__ double red[(1 << (B - R)) / 64];
+  __shared__ cx<T> gsh[(1 << (B - R)) / 64 * kDiagSlots];
   const StateTile st = state_tile(tile_log);
   const ProbeSink sink{probes + st.b * probes_per_state * st.ntiles, st.t, st.ntiles, false};
   reg_tile<T, B, R, 16, FULL>(sv + st.b * state_stride, pd + st.b * pd_stride, rops, tops, pool,
-                              sink, st.t, lds, mat, doff, red);
+                              sink, st.t, lds, mat, doff, red, gsh);
 }
 
 }  // namespace aerhip
