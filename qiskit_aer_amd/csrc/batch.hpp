@@ -52,6 +52,8 @@ __attribute__((amdgpu_waves_per_eu(reg_pass_waves<T, B, R>()))) void k_batch(cx<
     if (pd->kind == 2) {
       for (uint64_t t = 0; t < ntiles; ++t)
         reg_tile<T, B, R, 16, true>(sv, pd, rops, tops, pool, sink, t, lds, mat, doff, red, gsh);
+    } else if (pd->kind == 4) {
+      for (uint64_t t = 0; t < ntiles; ++t) qperm_tile<T, B - 1, kThreads>(sv, pd, tops + pd->op_begin, t, lds);
     } else if (pd->kind == 3) {
       const TileOp &op = tops[pd->op_begin];
       for (uint64_t t = 0; t < ntiles; ++t) {

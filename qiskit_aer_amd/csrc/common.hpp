@@ -90,6 +90,11 @@ enum TileOpKind : uint32_t {
   K_PROBE = 9,    // read-only: tile share of Re<psi|phase P|psi> into probe slot npairs
   K_PHASES = 10,  // fused diagonal of register passes: where the controls hold, the product
                   // over npairs members of (bit ? v : u); pool[off + 3m .. +2] = (qubit, u, v)
+  K_QPERM = 11,   // a layer of SWAPs on disjoint qubit pairs (an involution of the index
+                  // bits) as one pass: k pairs (pos[j], gpos[j]) among the qubits outside
+                  // the tile; cg: local bit i goes to local bit (cg >> 4i) & 15; bytes of
+                  // xl (j = 0..3) and zl (j = 4): the local bit XORed into LDS slot bit j
+                  // (0xFF: none)
 };
 
 constexpr uint8_t kNotLocal = 0xFF;
@@ -114,7 +119,8 @@ struct TileOp {
 };
 
 struct PassDesc {
-  uint32_t kind;      // 0 = tile pass, 1 = dense group pass
+  uint32_t kind;      // 0 = tile pass, 1 = dense group pass, 2 = register pass,
+                      // 3 = probe sweep, 4 = qubit-permutation pass (K_QPERM)
   uint32_t nb;        // tile qubits
   uint32_t c;         // S contains qubits 0..c-1
   uint32_t op_begin;  // first TileOp of the pass
@@ -123,7 +129,7 @@ struct PassDesc {
   uint32_t readonly;  // 1: the pass only reads the state (probes); nothing is written back
   uint32_t pad;
   uint8_t spos[16];   // tile qubits, ascending
-  uint8_t comp[64];   // qubits outside the tile, ascending
+  uint8_t comp[64];   // qubits outside the tile, ascending (kind 4: in pair order)
 };
 
 }  // namespace aerhip

@@ -1704,6 +1704,118 @@ double block_cost(uint32_t kind, uint32_t k, bool single) {
 
 }  // namespace
 
+// A run of SWAPs without controls on disjoint qubit pairs (a layer, such as
+// the final reversal of a QFT) whose qubits outside the low ones are more than
+// one tile pass can hold becomes one K_QPERM primitive: a single pass that
+// permutes the index bits, where the SWAPs one by one would take a tile pass
+// per few pairs.  At most 16 pairs per layer (the TileOp's pair list).
+std::vector<Prim> Engine::merge_swap_layers(const std::vector<Prim> &in) const {
+  const uint64_t lowmask = (1ull << low_bits_) - 1ull;
+  // the low qubits and their partners must fit the permutation tile
+  if (tile_bits_ < 2 * low_bits_ + 1) return in;
+  std::vector<Prim> out;
+  out.reserve(in.size());
+  size_t i = 0;
+  while (i < in.size()) {
+    size_t j = i;
+    uint64_t used = 0;
+    while (j < in.size() && j - i < 16 && in[j].kind == K_MCSWAP && in[j].ctrl.empty() &&
+           !(in[j].tmask & used)) {
+      used |= in[j].tmask;
+      ++j;
+    }
+    if (j - i >= 2 &&
+        static_cast<uint32_t>(__builtin_popcountll(used & ~lowmask)) > tile_bits_ - low_bits_) {
+      Prim p;
+      p.kind = K_QPERM;
+      p.solo = true;
+      p.tmask = p.touch = used;
+      for (size_t k = i; k < j; ++k) {
+        p.q.push_back(in[k].q[0]);
+        p.q.push_back(in[k].q[1]);
+      }
+      out.push_back(std::move(p));
+      i = j;
+    } else {
+      out.push_back(in[i]);
+      ++i;
+    }
+  }
+  return out;
+}
+
+// The pass of a K_QPERM primitive: tile qubits S closed under the pair
+// exchange, holding the low qubits and their partners (so that both tiles of
+// a pair are read and written in runs of 2^low_bits amplitudes), then the
+// lowest other qubits with their partners, up to tile_bits - 1 qubits (two
+// tiles share the workgroup's LDS).
+PassDesc Engine::qperm_pass(const Prim &p, std::vector<TileOp> &ops) const {
+  uint32_t partner[64];
+  for (uint32_t q = 0; q < 64; ++q) partner[q] = q;
+  for (size_t k = 0; k + 1 < p.q.size(); k += 2) {
+    partner[p.q[k]] = p.q[k + 1];
+    partner[p.q[k + 1]] = p.q[k];
+  }
+  const uint32_t sb = tile_bits_ - 1;
+  uint64_t S = 0;
+  for (uint32_t q = 0; q < low_bits_; ++q) S |= bit(q) | bit(partner[q]);
+  for (uint32_t q = low_bits_; q < n_; ++q) {
+    const uint64_t orbit = bit(q) | bit(partner[q]);
+    if (!(S & orbit) && static_cast<uint32_t>(__builtin_popcountll(S | orbit)) <= sb) S |= orbit;
+  }
+  if (static_cast<uint32_t>(__builtin_popcountll(S)) > sb) throw std::logic_error("aerhip: permutation tile");
+  PassDesc pd;
+  std::memset(&pd, 0, sizeof(pd));
+  int loc[64];
+  for (int &v : loc) v = -1;
+  uint32_t ns = 0, nc = 0;
+  for (uint32_t q = 0; q < n_; ++q) {
+    if (S & bit(q)) {
+      loc[q] = static_cast<int>(ns);
+      pd.spos[ns++] = static_cast<uint8_t>(q);
+    }
+  }
+  // Tile index bits in pair order: each qubit outside the tile followed by
+  // its partner.
+  for (uint32_t q = 0; q < n_; ++q) {
+    if ((S & bit(q)) || partner[q] < q) continue;
+    pd.comp[nc++] = static_cast<uint8_t>(q);
+    if (partner[q] != q) pd.comp[nc++] = static_cast<uint8_t>(partner[q]);
+  }
+  pd.kind = 4;
+  pd.nb = ns;
+  pd.ncomp = nc;
+  while (pd.c < ns && pd.spos[pd.c] == pd.c) ++pd.c;
+  pd.op_begin = static_cast<uint32_t>(ops.size());
+  pd.nops = 1;
+  TileOp op;
+  std::memset(&op, 0, sizeof(op));
+  op.kind = K_QPERM;
+  for (uint32_t i = 0; i < ns; ++i)
+    op.cg |= static_cast<uint64_t>(loc[partner[pd.spos[i]]]) << (4 * i);
+  // LDS order: local index k at k ^ g(k), g(k) bit j = bit P_j of k where the
+  // lane bit j of the reads lands (P_j = image of local bit j) if that is
+  // above the bank slot bits (16-byte slots of c128: 4 bits, 8-byte of c64:
+  // 5), so that the permuted reads, like the writes, meet distinct banks.
+  const uint32_t slot_bits = prec_ == AERHIP_PRECISION_DOUBLE ? 4 : 5;
+  op.xl = op.zl = 0xFFFFFFFFu;
+  for (uint32_t j = 0; j < slot_bits && j < ns; ++j) {
+    const uint32_t pj = static_cast<uint32_t>(loc[partner[pd.spos[j]]]);
+    if (pj < slot_bits) continue;
+    uint32_t &w = j < 4 ? op.xl : op.zl;
+    const uint32_t sh = 8 * (j & 3);
+    w = (w & ~(0xFFu << sh)) | (pj << sh);
+  }
+  for (size_t k = 0; k + 1 < p.q.size(); k += 2) {
+    if (S & bit(p.q[k])) continue;  // exchanged inside the tile
+    op.pos[op.k] = static_cast<uint8_t>(p.q[k]);
+    op.gpos[op.k] = static_cast<uint8_t>(p.q[k + 1]);
+    ++op.k;
+  }
+  ops.push_back(op);
+  return pd;
+}
+
 std::vector<Prim> Engine::fuse_blocks(const std::vector<Prim> &in) {
   const uint32_t width = block_width_;
   const bool single = prec_ == AERHIP_PRECISION_SINGLE;
@@ -1786,6 +1898,7 @@ void Engine::set_block_fusion(uint32_t width, bool cost_check) {
 
 void Engine::emit_program(Program &prog) {
   if (block_width_ >= 2 && n_ > fusion_threshold_) queue_ = fuse_blocks(queue_);
+  queue_ = merge_swap_layers(queue_);
   const std::vector<PassPlan> plans = plan();
   std::vector<PassDesc> &passes = prog.passes;
   std::vector<TileOp> &ops = prog.ops;
@@ -1794,6 +1907,10 @@ void Engine::emit_program(Program &prog) {
   prog.pool.insert(prog.pool.end(), pool_.begin(), pool_.end());
   passes.reserve(passes.size() + plans.size());
   for (const PassPlan &pp : plans) {
+    if (pp.solo && queue_[pp.prims[0]].kind == K_QPERM) {
+      passes.push_back(qperm_pass(queue_[pp.prims[0]], ops));
+      continue;
+    }
     PassDesc pd;
     std::memset(&pd, 0, sizeof(pd));
     int loc[64];
@@ -1997,6 +2114,15 @@ void launch_pass(int prec, uint32_t reg_bits, uint32_t n, uint32_t tile_bits, co
       k_probe_sweep<float><<<grid_of(tl), kThreads, 0, stream>>>(
           static_cast<const cx<float> *>(sv), state_stride, dpd, dp.ops, dp.probes, state_stride, pd_stride,
           tl, pps);
+  } else if (pd.kind == 4) {
+    // qubit permutation (a layer of SWAPs): one workgroup per pair of tiles
+    const uint32_t tl = pd.ncomp;
+    if (prec == AERHIP_PRECISION_DOUBLE)
+      k_qperm<double, kTileBitsC128 - 1><<<grid_of(tl), kThreads, 0, stream>>>(
+          static_cast<cx<double> *>(sv), dpd, dp.ops, state_stride, pd_stride, tl);
+    else
+      k_qperm<float, kTileBitsC64 - 1><<<grid_of(tl), kThreads, 0, stream>>>(
+          static_cast<cx<float> *>(sv), dpd, dp.ops, state_stride, pd_stride, tl);
   } else if (pd.kind == 0) {
     // LDS-resident tile pass (tiles smaller than the register layouts need,
     // or register layouts switched off).

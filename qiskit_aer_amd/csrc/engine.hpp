@@ -163,6 +163,8 @@ class Engine {
   void apply_diagonal_1(uint32_t q, const cd *diag);
   std::vector<PassPlan> plan() const;
   std::vector<Prim> fuse_blocks(const std::vector<Prim> &in);
+  std::vector<Prim> merge_swap_layers(const std::vector<Prim> &in) const;
+  PassDesc qperm_pass(const Prim &p, std::vector<TileOp> &ops) const;
   void ensure_partial(size_t n);
   void *scratch(size_t bytes);
   double fetch_scalar(const double *dptr);

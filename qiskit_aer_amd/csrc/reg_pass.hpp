@@ -1218,4 +1218,107 @@ __attribute__((amdgpu_waves_per_eu(reg_pass_waves<T, B, R, WIDE>()))) void k_reg
                               sink, st.t, lds, mat, doff, red, gsh);
 }
 
+// A layer of SWAPs on disjoint qubit pairs permutes the index bits by an
+// involution pi: amplitude i moves to pi(i).  The tile qubits S are closed
+// under pi, so tile (S, b) maps onto tile (S, pi(b)) with its local bits
+// permuted; one workgroup exchanges such a pair of tiles (or permutes a tile
+// with pi(b) = b in place), and the other tile's workgroup has nothing to do,
+// so the pass is in place without races.  Both tiles go through LDS: 2^SB
+// amplitudes each.
+template <typename T, int SB, int NT>
+__device__ __forceinline__ void qperm_tile(cx<T> *__restrict__ sv, const PassDesc *__restrict__ pdp,
+                                           const TileOp *__restrict__ opp, uint64_t t, cx<T> *lds) {
+  const uint32_t tid = threadIdx.x;
+  // The descriptors, read once into scalar registers (compile-time loop
+  // bounds keep them there).
+  const uint32_t ns = pdp->nb, npairs = opp->k;
+  const uint64_t cg = opp->cg;
+  const uint32_t xl = opp->xl, zl = opp->zl;
+  uint32_t sp[SB], px[16], py[16];
+#pragma unroll
+  for (int i = 0; i < SB; ++i) sp[i] = pdp->spos[i];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    px[j] = opp->pos[j];
+    py[j] = opp->gpos[j];
+  }
+  // base of tile t: bit j of t at qubit comp[j] (pair order: each qubit
+  // outside the tile next to its partner, so that the workgroups running
+  // together read and write the same regions of the state)
+  const uint32_t nc = pdp->ncomp;
+  uint32_t cw[16];
+#pragma unroll
+  for (int w = 0; w < 16; ++w) cw[w] = reinterpret_cast<const uint32_t *>(pdp->comp)[w];
+  uint64_t b = 0;
+#pragma unroll
+  for (int j = 0; j < 64; ++j)
+    if (static_cast<uint32_t>(j) < nc) b |= ((t >> j) & 1ull) << ((cw[j >> 2] >> (8 * (j & 3))) & 0xFFu);
+  uint64_t pb = b;
+#pragma unroll
+  for (int j = 0; j < 16; ++j)
+    if (static_cast<uint32_t>(j) < npairs && (((b >> px[j]) ^ (b >> py[j])) & 1ull))
+      pb ^= (1ull << px[j]) | (1ull << py[j]);
+  b = uniform64(b);
+  pb = uniform64(pb);
+  if (pb < b) return;  // the workgroup of tile pb exchanges the pair (uniform)
+  const uint32_t N = 1u << ns;
+  // The maps below are linear in the local index: the thread's part once,
+  // the part of each step (k = tid + k0, k0 a multiple of NT) in the loop.
+  auto expand = [&](uint32_t k) {
+    uint64_t e = 0;
+#pragma unroll
+    for (int i = 0; i < SB; ++i)
+      if (static_cast<uint32_t>(i) < ns) e |= static_cast<uint64_t>((k >> i) & 1u) << sp[i];
+    return e;
+  };
+  auto lperm = [&](uint32_t k) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int i = 0; i < SB; ++i)
+      if (static_cast<uint32_t>(i) < ns) m |= ((k >> i) & 1u) << ((cg >> (4 * i)) & 15u);
+    return m;
+  };
+  // LDS slot of local index k: k ^ g(k) (bank-conflict-free permuted reads)
+  auto swz = [&](uint32_t k) {
+    uint32_t g = 0;
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+      const uint32_t p = ((j < 4 ? xl : zl) >> (8 * (j & 3))) & 0xFFu;
+      if (p < 32) g |= ((k >> p) & 1u) << j;
+    }
+    return k ^ g;
+  };
+  const uint32_t tk = tid & (N - 1u);
+  const uint64_t et = expand(tk);
+  const uint32_t wt = swz(tk), rt = swz(lperm(tk));
+  const bool pair = pb != b;
+  for (uint32_t k0 = 0; k0 < N; k0 += NT) {
+    if (tid >= N) break;
+    const uint64_t e = et | expand(k0);
+    const uint32_t w = wt ^ swz(k0);
+    lds[w] = ld_stream(sv + b + e);
+    if (pair) lds[N + w] = ld_stream(sv + pb + e);
+  }
+  __syncthreads();
+  for (uint32_t k0 = 0; k0 < N; k0 += NT) {
+    if (tid >= N) break;
+    const uint64_t e = et | expand(k0);
+    const uint32_t r = rt ^ swz(lperm(k0));
+    st_stream(sv + pb + e, lds[r]);
+    if (pair) st_stream(sv + b + e, lds[N + r]);
+  }
+  __syncthreads();  // the LDS tile is free again
+}
+
+// One workgroup per tile pair (the workgroup of the larger base returns).
+template <typename T, int SB>
+__global__ __launch_bounds__(kThreads) void k_qperm(cx<T> *__restrict__ sv, const PassDesc *__restrict__ pd,
+                                                    const TileOp *__restrict__ ops, uint64_t state_stride,
+                                                    uint32_t pd_stride, uint32_t tile_log) {
+  __shared__ cx<T> lds[2 << SB];
+  const StateTile st = state_tile(tile_log);
+  const PassDesc *d = pd + st.b * pd_stride;
+  qperm_tile<T, SB, kThreads>(sv + st.b * state_stride, d, ops + d->op_begin, st.t, lds);
+}
+
 }  // namespace aerhip

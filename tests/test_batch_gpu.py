@@ -59,3 +59,21 @@ def test_batch_edge_cases():
     assert abs(e[0] - 2.0) < 1e-12
     assert abs(e[1] - 1.0) < 1e-12
     assert e[2] == 0.0
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_batch_with_swap_layers(precision):
+    """Circuits ending in a qubit reversal (a SWAP layer run as one
+    bit-permutation pass) inside the batched launch."""
+    sizes = [6, 12, 13, 16]
+    print(f"case: batch swap layers {precision} sizes={sizes}", flush=True)
+    cs, hs = workload(sizes, 11)
+    for c in cs:
+        n = c.num_qubits
+        for q in range(n // 2):
+            c.gate("swap", [q, n - 1 - q])
+        c.gate("h", [0])
+    e_batch, _ = batch_pauli_expval(cs, hs, precision, "batch")
+    want = np.array([oracle_energy(c, h) for c, h in zip(cs, hs)])
+    tol = 1e-10 if precision == "double" else 1e-4
+    assert np.max(np.abs(e_batch - want)) < tol

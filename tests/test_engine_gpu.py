@@ -634,3 +634,42 @@ def test_qft_fused_vs_unfused_vs_dft(precision):
         e.apply_ops(circuits.qft(n))
         assert np.max(np.abs(e.copy_to_vector() - want)) < tol, fuse
         assert (e.fused_count() > 0) == fuse
+
+
+def swap_layer(n, k, rng):
+    qs = [int(x) for x in rng.permutation(n)[:2 * k]]
+    return [(qs[2 * i], qs[2 * i + 1]) for i in range(k)]
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+@pytest.mark.parametrize("n", [9, 13, 20])
+def test_swap_layers_as_permutation_passes(precision, n):
+    """Layers of SWAPs on disjoint pairs (merged into one bit-permutation pass
+    when they span more qubits than a tile pass holds): bit-exact against the
+    oracle's SWAPs one by one, alone and between gates; the reversal of all
+    qubits (the QFT's final layer) included."""
+    print(f"case: swap layers n={n} {precision}", flush=True)
+    e, o, rng = pair(n, 40 + n, precision)
+    if precision == "single":
+        o.initialize_from_vector(e.copy_to_vector().astype(np.complex128))
+    layers = [swap_layer(n, k, rng) for k in (2, min(16, n // 2), min(7, n // 2))]
+    layers.append([(q, n - 1 - q) for q in range(n // 2)])
+    for layer in layers:
+        for a, b in layer:
+            e.apply_mcswap([a, b])
+            o.apply_mcswap([a, b])
+    assert np.array_equal(e.copy_to_vector(), o.copy_to_vector().astype(e.copy_to_vector().dtype))
+    # between gates, with controlled swaps (never merged) in the run
+    for layer in layers:
+        q = int(rng.integers(n))
+        m = D.u3(*rng.uniform(0, 2 * np.pi, 3))
+        e.apply_mcu([q], m)
+        o.apply_mcu([q], m)
+        for i, (a, b) in enumerate(layer):
+            e.apply_mcswap([a, b])
+            o.apply_mcswap([a, b])
+            if i == 1:
+                c = [x for x in range(n) if x not in (a, b)][0]
+                e.apply_mcswap([c, a, b])
+                o.apply_mcswap([c, a, b])
+    close(e, o, precision)
