@@ -129,7 +129,7 @@ struct PassDesc {
   uint32_t readonly;  // 1: the pass only reads the state (probes); nothing is written back
   uint32_t pad;
   uint8_t spos[16];   // tile qubits, ascending
-  uint8_t comp[64];   // qubits outside the tile, ascending (kind 4: in pair order)
+  uint8_t comp[64];   // qubits outside the tile, ascending
 };
 
 }  // namespace aerhip

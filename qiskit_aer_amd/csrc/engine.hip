@@ -1773,14 +1773,9 @@ PassDesc Engine::qperm_pass(const Prim &p, std::vector<TileOp> &ops) const {
     if (S & bit(q)) {
       loc[q] = static_cast<int>(ns);
       pd.spos[ns++] = static_cast<uint8_t>(q);
+    } else {
+      pd.comp[nc++] = static_cast<uint8_t>(q);
     }
-  }
-  // Tile index bits in pair order: each qubit outside the tile followed by
-  // its partner.
-  for (uint32_t q = 0; q < n_; ++q) {
-    if ((S & bit(q)) || partner[q] < q) continue;
-    pd.comp[nc++] = static_cast<uint8_t>(q);
-    if (partner[q] != q) pd.comp[nc++] = static_cast<uint8_t>(partner[q]);
   }
   pd.kind = 4;
   pd.nb = ns;
@@ -2115,14 +2110,15 @@ void launch_pass(int prec, uint32_t reg_bits, uint32_t n, uint32_t tile_bits, co
           static_cast<const cx<float> *>(sv), state_stride, dpd, dp.ops, dp.probes, state_stride, pd_stride,
           tl, pps);
   } else if (pd.kind == 4) {
-    // qubit permutation (a layer of SWAPs): one workgroup per pair of tiles
-    const uint32_t tl = pd.ncomp;
+    // qubit permutation (a layer of SWAPs): four tiles per workgroup, each
+    // exchanged with its partner tile or skipped
+    const uint32_t per = std::min<uint32_t>(2, pd.ncomp), tl = pd.ncomp - per;
     if (prec == AERHIP_PRECISION_DOUBLE)
       k_qperm<double, kTileBitsC128 - 1><<<grid_of(tl), kThreads, 0, stream>>>(
-          static_cast<cx<double> *>(sv), dpd, dp.ops, state_stride, pd_stride, tl);
+          static_cast<cx<double> *>(sv), dpd, dp.ops, state_stride, pd_stride, tl, per);
     else
       k_qperm<float, kTileBitsC64 - 1><<<grid_of(tl), kThreads, 0, stream>>>(
-          static_cast<cx<float> *>(sv), dpd, dp.ops, state_stride, pd_stride, tl);
+          static_cast<cx<float> *>(sv), dpd, dp.ops, state_stride, pd_stride, tl, per);
   } else if (pd.kind == 0) {
     // LDS-resident tile pass (tiles smaller than the register layouts need,
     // or register layouts switched off).

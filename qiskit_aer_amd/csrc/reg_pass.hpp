@@ -1242,17 +1242,14 @@ __device__ __forceinline__ void qperm_tile(cx<T> *__restrict__ sv, const PassDes
     px[j] = opp->pos[j];
     py[j] = opp->gpos[j];
   }
-  // base of tile t: bit j of t at qubit comp[j] (pair order: each qubit
-  // outside the tile next to its partner, so that the workgroups running
-  // together read and write the same regions of the state)
-  const uint32_t nc = pdp->ncomp;
-  uint32_t cw[16];
+  // base of tile t: t with zeros inserted at the tile qubits (ascending)
+  uint64_t b = t;
 #pragma unroll
-  for (int w = 0; w < 16; ++w) cw[w] = reinterpret_cast<const uint32_t *>(pdp->comp)[w];
-  uint64_t b = 0;
-#pragma unroll
-  for (int j = 0; j < 64; ++j)
-    if (static_cast<uint32_t>(j) < nc) b |= ((t >> j) & 1ull) << ((cw[j >> 2] >> (8 * (j & 3))) & 0xFFu);
+  for (int i = 0; i < SB; ++i)
+    if (static_cast<uint32_t>(i) < ns) {
+      const uint32_t q = sp[i];
+      b = ((b >> q) << (q + 1)) | (b & ((1ull << q) - 1ull));
+    }
   uint64_t pb = b;
 #pragma unroll
   for (int j = 0; j < 16; ++j)
@@ -1310,15 +1307,19 @@ __device__ __forceinline__ void qperm_tile(cx<T> *__restrict__ sv, const PassDes
   __syncthreads();  // the LDS tile is free again
 }
 
-// One workgroup per tile pair (the workgroup of the larger base returns).
+// 2^per_wg_log consecutive tiles per workgroup (about half of them are the
+// larger base of their pair and skipped, so one tile per workgroup would
+// leave half the workgroups idle after their set-up).
 template <typename T, int SB>
 __global__ __launch_bounds__(kThreads) void k_qperm(cx<T> *__restrict__ sv, const PassDesc *__restrict__ pd,
                                                     const TileOp *__restrict__ ops, uint64_t state_stride,
-                                                    uint32_t pd_stride, uint32_t tile_log) {
+                                                    uint32_t pd_stride, uint32_t tile_log,
+                                                    uint32_t per_wg_log) {
   __shared__ cx<T> lds[2 << SB];
   const StateTile st = state_tile(tile_log);
   const PassDesc *d = pd + st.b * pd_stride;
-  qperm_tile<T, SB, kThreads>(sv + st.b * state_stride, d, ops + d->op_begin, st.t, lds);
+  for (uint32_t i = 0; i < (1u << per_wg_log); ++i)
+    qperm_tile<T, SB, kThreads>(sv + st.b * state_stride, d, ops + d->op_begin, (st.t << per_wg_log) + i, lds);
 }
 
 }  // namespace aerhip
