@@ -38,7 +38,7 @@ __attribute__((amdgpu_waves_per_eu(reg_pass_waves<T, B, R>()))) void k_batch(cx<
   __shared__ cx<T> mat[16];
   __shared__ uint32_t doff[32];
   __shared__ double red[kThreads / 64];
-  __shared__ cx<T> gsh[kThreads / 64 * kDiagSlots];
+  __shared__ cx<double> dlt[kDiagSlots * 32];
   const BatchCircuit c = circ[blockIdx.x];
   cx<T> *sv = states + c.sv_off;
   const uint64_t size = 1ull << c.num_qubits;
@@ -51,7 +51,7 @@ __attribute__((amdgpu_waves_per_eu(reg_pass_waves<T, B, R>()))) void k_batch(cx<
     const uint64_t ntiles = 1ull << pd->ncomp;
     if (pd->kind == 2) {
       for (uint64_t t = 0; t < ntiles; ++t)
-        reg_tile<T, B, R, 16, true>(sv, pd, rops, tops, pool, sink, t, lds, mat, doff, red, gsh);
+        reg_tile<T, B, R, 16, true>(sv, pd, rops, tops, pool, sink, t, lds, mat, doff, red, dlt);
     } else if (pd->kind == 4) {
       for (uint64_t t = 0; t < ntiles; ++t) qperm_tile<T, B - 1, kThreads>(sv, pd, tops + pd->op_begin, t, lds);
     } else if (pd->kind == 3) {

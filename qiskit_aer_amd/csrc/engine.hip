@@ -1,5 +1,6 @@
 // Host implementation of the engine and its C ABI (include/aerhip.h).
 #include <algorithm>
+#include <array>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -1141,9 +1142,10 @@ void register_program(const std::vector<TileOp> &in, uint32_t B, uint32_t R, uin
     map_dirty = false;
   };
   // Fused diagonals of the phase that ends: slot k for the k-th (up to
-  // kDiagSlots), lane record 8k + c for its table c, after an entry equal to 1
-  // (the table of unused lanes); table distances are counted back from the
-  // first record, so that moving the pool moves nothing else.
+  // kDiagSlots), lane record 8k + c for its table c and staging distance
+  // 32k + e for entry e of its lane tables, after an entry equal to 1 (the
+  // table of unused lanes); distances are counted back from the first record,
+  // so that moving the pool moves nothing else.
   auto assign_diag_slots = [&] {
     const uint32_t NR = 1u << R;
     std::vector<ROp<double> *> df;
@@ -1152,7 +1154,7 @@ void register_program(const std::vector<TileOp> &in, uint32_t B, uint32_t R, uin
     if (df.empty()) return;
     pool.push_back(cd(1.0, 0.0));
     const size_t first = pool.size();
-    if (first + 32 > 0xFFFFFFFFull) throw std::length_error("aerhip: pool offset");
+    if (first + 96 > 0xFFFFFFFFull) throw std::length_error("aerhip: pool offset");
     DiagLane lanes[64];
     for (uint32_t l = 0; l < 64; ++l) {
       lanes[l].back = 1;
@@ -1163,9 +1165,15 @@ void register_program(const std::vector<TileOp> &in, uint32_t B, uint32_t R, uin
       lanes[l].fq = 0;
       for (uint32_t w = 0; w < 4; ++w) lanes[l].fq |= static_cast<uint32_t>(df[k]->u.fq[4 * c + w]) << (8 * w);
     }
-    cd w[32];
+    uint32_t stage[32 * kDiagSlots];
+    for (uint32_t i = 0; i < 32 * kDiagSlots; ++i) {
+      const uint32_t k = i / 32, e = i % 32;
+      stage[i] = k < df.size() ? static_cast<uint32_t>(first - (df[k]->extra + NR + e)) : 1u;
+    }
+    cd w[32 + 64];
     std::memcpy(w, lanes, sizeof(lanes));
-    pool.insert(pool.end(), w, w + 32);
+    std::memcpy(w + 32, stage, sizeof(stage));
+    pool.insert(pool.end(), w, w + 96);
     for (size_t k = 0; k < df.size(); ++k) df[k]->a = (df[k]->a & 0xFFu) | static_cast<uint32_t>(k + 1) << 16;
     ROp<double> &e = rops[enter_at];
     e.a = static_cast<uint32_t>(df.size());
@@ -2046,29 +2054,33 @@ struct DeviceProgram {
 };
 
 template <typename T, int B, int R>
-void launch_reg_pass(bool lean, bool wide, uint32_t grid, hipStream_t stream, cx<T> *sv, const PassDesc *pd,
-                     const ROp<T> *rops, const TileOp *tops, const cx<double> *pool, double *probes,
-                     uint64_t state_stride, uint32_t pd_stride, uint32_t tile_log, uint32_t pps) {
+void launch_reg_pass(bool lean, bool wide, bool diag, uint32_t grid, hipStream_t stream, cx<T> *sv,
+                     const PassDesc *pd, const ROp<T> *rops, const TileOp *tops, const cx<double> *pool,
+                     double *probes, uint64_t state_stride, uint32_t pd_stride, uint32_t tile_log,
+                     uint32_t pps) {
+  // the staged lane tables of fused diagonals, only for passes that have them
+  const size_t dyn = diag ? kDiagLdsBytes : 0;
   if constexpr (sizeof(T) == 8 && R == 3) {
     if (lean && wide) {
-      k_reg_pass<T, B, R, false, true><<<grid, 1 << (B - R), 0, stream>>>(
+      k_reg_pass<T, B, R, false, true><<<grid, 1 << (B - R), dyn, stream>>>(
           sv, pd, rops, tops, pool, probes, state_stride, pd_stride, tile_log, pps);
       return;
     }
   }
   if (lean)
-    k_reg_pass<T, B, R, false><<<grid, 1 << (B - R), 0, stream>>>(sv, pd, rops, tops, pool, probes,
-                                                                  state_stride, pd_stride, tile_log, pps);
+    k_reg_pass<T, B, R, false><<<grid, 1 << (B - R), dyn, stream>>>(sv, pd, rops, tops, pool, probes,
+                                                                    state_stride, pd_stride, tile_log, pps);
   else
-    k_reg_pass<T, B, R, true><<<grid, 1 << (B - R), 0, stream>>>(sv, pd, rops, tops, pool, probes,
-                                                                 state_stride, pd_stride, tile_log, pps);
+    k_reg_pass<T, B, R, true><<<grid, 1 << (B - R), dyn, stream>>>(sv, pd, rops, tops, pool, probes,
+                                                                   state_stride, pd_stride, tile_log, pps);
 }
 
 // Launches pass index i (host descriptor `pd`, the same pass of every state)
-// over `count` states; `lean` / `wide` pick the register-pass variant.
+// over `count` states; `lean` / `wide` pick the register-pass variant, `diag`
+// gives it the LDS of staged fused-diagonal lane tables.
 void launch_pass(int prec, uint32_t reg_bits, uint32_t n, uint32_t tile_bits, const PassDesc &pd,
-                 uint32_t i, bool lean, bool wide, const DeviceProgram &dp, void *sv, uint64_t count,
-                 uint32_t pd_stride, hipStream_t stream) {
+                 uint32_t i, bool lean, bool wide, bool diag, const DeviceProgram &dp, void *sv,
+                 uint64_t count, uint32_t pd_stride, hipStream_t stream) {
   const uint64_t state_stride = 1ull << n;
   const PassDesc *dpd = dp.passes + i;
   const uint32_t pps = dp.probes_per_state;
@@ -2083,19 +2095,19 @@ void launch_pass(int prec, uint32_t reg_bits, uint32_t n, uint32_t tile_bits, co
       auto *s = static_cast<cx<double> *>(sv);
       auto *ro = static_cast<const ROp<double> *>(dp.rops);
       if (reg_bits == 3)
-        launch_reg_pass<double, kTileBitsC128, 3>(lean, wide, grid_of(tl), stream, s, dpd, ro, dp.ops, dp.pool,
+        launch_reg_pass<double, kTileBitsC128, 3>(lean, wide, diag, grid_of(tl), stream, s, dpd, ro, dp.ops, dp.pool,
                                                   dp.probes, state_stride, pd_stride, tl, pps);
       else
-        launch_reg_pass<double, kTileBitsC128, 4>(lean, wide, grid_of(tl), stream, s, dpd, ro, dp.ops, dp.pool,
+        launch_reg_pass<double, kTileBitsC128, 4>(lean, wide, diag, grid_of(tl), stream, s, dpd, ro, dp.ops, dp.pool,
                                                   dp.probes, state_stride, pd_stride, tl, pps);
     } else {
       auto *s = static_cast<cx<float> *>(sv);
       auto *ro = static_cast<const ROp<float> *>(dp.rops);
       if (reg_bits == 4)
-        launch_reg_pass<float, kTileBitsC64, 4>(lean, wide, grid_of(tl), stream, s, dpd, ro, dp.ops, dp.pool,
+        launch_reg_pass<float, kTileBitsC64, 4>(lean, wide, diag, grid_of(tl), stream, s, dpd, ro, dp.ops, dp.pool,
                                                 dp.probes, state_stride, pd_stride, tl, pps);
       else
-        launch_reg_pass<float, kTileBitsC64, 5>(lean, wide, grid_of(tl), stream, s, dpd, ro, dp.ops, dp.pool,
+        launch_reg_pass<float, kTileBitsC64, 5>(lean, wide, diag, grid_of(tl), stream, s, dpd, ro, dp.ops, dp.pool,
                                                 dp.probes, state_stride, pd_stride, tl, pps);
     }
   } else if (pd.kind == 3) {
@@ -2157,11 +2169,18 @@ bool graphs_enabled() {
 
 // The register-pass variant of a pass: lean if every op is a lean kind, wide
 // if it also has at most kWidePassOps ops.
-void pass_variant(const PassDesc &pd, const std::vector<ROp<double>> &rops, bool &lean, bool &wide) {
+void pass_variant(const PassDesc &pd, const std::vector<ROp<double>> &rops, bool &lean, bool &wide,
+                  bool &diag) {
   lean = true;
   wide = pd.nops <= kWidePassOps;
+  diag = false;
   if (pd.kind != 2) return;
-  for (uint32_t k = 0; k < pd.nops && lean; ++k) lean = lean_kind(rops[pd.op_begin + k].kind);
+  for (uint32_t k = 0; k < pd.nops; ++k) {
+    const ROp<double> &r = rops[pd.op_begin + k];
+    lean = lean && lean_kind(r.kind);
+    const uint32_t kk = r.kind & ~F_ALL;
+    diag = diag || ((kk == Q_LOAD || kk == Q_FILL) && r.a != 0);
+  }
 }
 }  // namespace
 
@@ -2240,19 +2259,19 @@ void Engine::flush() {
   std::vector<uint64_t> sig = {reinterpret_cast<uint64_t>(d_prog_), reinterpret_cast<uint64_t>(sv_),
                                reinterpret_cast<uint64_t>(dprobes), static_cast<uint64_t>(prec_),
                                reg_bits_, n_, tile_bits_, passes.size(), ops_off, rops_off, pool_off};
-  std::vector<std::pair<bool, bool>> variant(passes.size());
+  std::vector<std::array<bool, 3>> variant(passes.size());
   for (size_t i = 0; i < passes.size(); ++i) {
-    bool lean = false, wide = false;
-    pass_variant(passes[i], rops, lean, wide);
-    variant[i] = {lean, wide};
+    bool lean = false, wide = false, diag = false;
+    pass_variant(passes[i], rops, lean, wide, diag);
+    variant[i] = {lean, wide, diag};
     const PassDesc &pd = passes[i];
     sig.push_back((uint64_t(pd.kind) << 48) | (uint64_t(pd.ncomp) << 32) | (uint64_t(pd.nb) << 16) |
-                  (uint64_t(lean) << 1) | uint64_t(wide));
+                  (uint64_t(diag) << 2) | (uint64_t(lean) << 1) | uint64_t(wide));
   }
   auto launch_all = [&] {
     for (size_t i = 0; i < passes.size(); ++i)
-      launch_pass(prec_, reg_bits_, n_, tile_bits_, passes[i], static_cast<uint32_t>(i), variant[i].first,
-                  variant[i].second, dp, sv_, 1, 0, stream_);
+      launch_pass(prec_, reg_bits_, n_, tile_bits_, passes[i], static_cast<uint32_t>(i), variant[i][0],
+                  variant[i][1], variant[i][2], dp, sv_, 1, 0, stream_);
   };
   if (graphs_enabled() && graph_exec_ && sig == graph_sig_) {
     AERHIP_CHECK(hipGraphLaunch(graph_exec_, stream_));
@@ -3004,16 +3023,17 @@ class MultiState {
     std::vector<TileOp> ops;
     std::vector<ROp<double>> rops;
     std::vector<cd> pool;
-    std::vector<char> lean(P, 1), wide(P, 1);
+    std::vector<char> lean(P, 1), wide(P, 1), diag(P, 0);
     for (uint64_t b = first; b < first + cnt; ++b) {
       const Program &p = parts[b];
       const uint32_t ob = static_cast<uint32_t>(ops.size()), rb = static_cast<uint32_t>(rops.size()),
                      pb = static_cast<uint32_t>(pool.size());
       for (size_t k = 0; k < P; ++k) {
-        bool l = false, w = false;
-        pass_variant(p.passes[k], p.rops, l, w);
+        bool l = false, w = false, dg = false;
+        pass_variant(p.passes[k], p.rops, l, w, dg);
         lean[k] = lean[k] && l;
         wide[k] = wide[k] && w;
+        diag[k] = diag[k] || dg;
         PassDesc d = p.passes[k];
         d.op_begin += d.kind == 2 ? rb : ob;
         passes.push_back(d);
@@ -3071,7 +3091,7 @@ class MultiState {
     void *sv = static_cast<char *>(states_) + first * (1ull << n_) * amp_bytes();
     for (size_t k = 0; k < P; ++k)
       launch_pass(prec_, planners_[first]->reg_bits(), n_, tb, parts[first].passes[k],
-                  static_cast<uint32_t>(k), lean[k] != 0, wide[k] != 0, dp, sv, cnt,
+                  static_cast<uint32_t>(k), lean[k] != 0, wide[k] != 0, diag[k] != 0, dp, sv, cnt,
                   static_cast<uint32_t>(P), st_);
     if (pps) {
       double *sums = d_probes_ + cnt * pps * ntiles;

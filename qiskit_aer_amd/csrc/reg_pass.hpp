@@ -57,7 +57,7 @@ enum RegPassOpKind : uint32_t {
                     // pool[extra..]: NR register factors, 16 + 16 lane factors (thread bits
                     // 0-3, 4-7), then a & 0xFF = number of 16-entry tables over fixed
                     // qubits u.fq[4c .. 4c+3] (0xFF: unused); a >> 16 = slot + 1 of the
-                    // phase's precomputed fixed-qubit factor (0: none, read the tables)
+                    // phase's lane tables staged in LDS (0: none, read the tables)
 };
 
 // Fused diagonals of a register phase: the factor over the qubits fixed by
@@ -65,17 +65,16 @@ enum RegPassOpKind : uint32_t {
 // kDiagSlots ops, while the tile loads are in flight, instead of as a chain of
 // dependent table loads inside each op.  Lane l of a wave reads table l % 8 of
 // slot l / 8, and the eight lanes of a slot multiply their entries together.
-// One 8-byte record per lane in the pool: the distance back from the first
-// record to the table (an entry equal to 1 just before the records for an
-// unused table or slot), and the table's four qubits (0xFF: unused).
+// The slots' lane tables (16 entries over thread bits 0-3, with that factor
+// folded in, and 16 over thread bits 4-7) are then staged in LDS, so that an
+// op reads its lane factor from LDS instead of through the vector caches.
+// In the pool, after an entry equal to 1: one 8-byte record per lane (the
+// distance back from the first record to the table, the entry equal to 1 for
+// an unused table or slot, and the table's four qubits, 0xFF: unused), then
+// for staging thread i = 32k + e the distance back from the first record to
+// entry e of slot k's lane tables (the entry equal to 1 past the last slot).
 constexpr uint32_t kDiagSlots = 8;
-// Where the phase keeps the slot factors: in a register (lane 8k: slot k)
-// for the lean complex128 variant, whose five-tile LDS budget has no room
-// left; elsewhere in LDS, one row per wave (no register to spare there).
-template <typename T, bool FULL>
-constexpr bool diag_in_lds() {
-  return !(sizeof(T) == 8 && !FULL);
-}
+constexpr uint32_t kDiagLdsBytes = kDiagSlots * 32 * sizeof(cx<double>);
 struct DiagLane {
   uint32_t back;
   uint32_t fq;
@@ -680,23 +679,6 @@ __device__ __forceinline__ uint64_t uniform64(uint64_t v) {
   return (static_cast<uint64_t>(hi) << 32) | lo;
 }
 
-// Lane l's value of a wave-uniform lane index.
-__device__ __forceinline__ cx<double> read_lane(const cx<double> &v, uint32_t l) {
-  auto rd = [l](double x) {
-    const uint64_t u = __builtin_bit_cast(uint64_t, x);
-    const uint32_t lo = __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(u)), static_cast<int>(l));
-    const uint32_t hi = __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(u >> 32)), static_cast<int>(l));
-    return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
-  };
-  return mk<double>(rd(v.x), rd(v.y));
-}
-__device__ __forceinline__ cx<float> read_lane(const cx<float> &v, uint32_t l) {
-  auto rd = [l](float x) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), static_cast<int>(l)));
-  };
-  return mk<float>(rd(v.x), rd(v.y));
-}
-
 // The value itself, hidden from the optimizer (no code is emitted).
 __device__ __forceinline__ uint32_t opaque_u32(uint32_t v) {
   asm volatile("" : "+s"(v));
@@ -823,8 +805,7 @@ __device__ __forceinline__ Amps<T, 1 << R> rw_op(const ROp<T> &op, const OpHead<
                                                  double *red, uint64_t base, uint32_t tid,
                                                  const TileOp *__restrict__ tops,
                                                  const cx<double> *__restrict__ pool,
-                                                 const ProbeSink &sink, const cx<T> &gdf,
-                                                 const cx<T> *gsh) {
+                                                 const ProbeSink &sink, const cx<double> *dlt) {
   constexpr int NR = 1 << R;
   constexpr int TB = B - R;
   const uint32_t kind = hd.kind & ~F_ALL, ra = hd.a, rb = hd.b, extra = hd.extra;
@@ -950,14 +931,13 @@ __device__ __forceinline__ Amps<T, 1 << R> rw_op(const ROp<T> &op, const OpHead<
     // whole pass
     uint32_t t = tid;
     asm volatile("" : "+v"(t));
-    cx<double> g = blk[NR + (t & 15u)];
-    g = cmul(g, blk[NR + 16 + ((t >> 4) & 15u)]);
+    cx<double> g;
     if (const uint32_t slot = ra >> 16) {
-      cx<T> gs;
-      if constexpr (diag_in_lds<T, FULL>()) gs = gsh[(t >> 6) * kDiagSlots + slot - 1u];
-      else gs = read_lane(gdf, 8u * (slot - 1u));
-      g = cmul(g, mk<double>(gs.x, gs.y));
+      // staged in LDS at the phase start, the fixed-qubit factor included
+      const cx<double> *row = dlt + 32u * (slot - 1u);
+      g = cmul(row[t & 15u], row[16u + ((t >> 4) & 15u)]);
     } else {
+      g = cmul(blk[NR + (t & 15u)], blk[NR + 16 + ((t >> 4) & 15u)]);
       const cx<double> *tab = blk + NR + 32;
       for (uint32_t c = 0; c < (ra & 0xFFu); ++c, tab += 16) {
         uint32_t idx = 0;
@@ -1084,6 +1064,29 @@ __device__ __forceinline__ cx<double> slot_factor(uint2 rec, const cx<double> *d
   return g;
 }
 
+// Stages the lane tables of the phase's fused-diagonal slots in LDS:
+// dlt[32k + e] = entry e of slot k's lane tables, entries 0-15 times the
+// slot's fixed-qubit factor (which lanes 8k..8k+7 of every wave hold), then
+// waits for the workgroup.  Staging thread i reads the entry at distance
+// sb[i / NT] back from the records (read before the tile loads).
+template <int NT>
+__device__ __forceinline__ void stage_diag_tables(uint2 drec, const cx<double> *dlane, uint32_t nslot,
+                                                  uint64_t base, uint32_t tid, const uint32_t (&sb)[256 / NT],
+                                                  cx<double> *dlt) {
+  const cx<double> g = slot_factor(drec, dlane, base);
+#pragma unroll
+  for (int j = 0; j < 256 / NT; ++j) {
+    const uint32_t i = static_cast<uint32_t>(j) * NT + tid, k = i >> 5;
+    const cx<double> gk = mk<double>(__shfl(g.x, 8 * static_cast<int>(k)), __shfl(g.y, 8 * static_cast<int>(k)));
+    if (i < 32u * nslot) {
+      cx<double> v = *(dlane - static_cast<int64_t>(sb[j]));
+      if ((i & 31u) < 16u) v = cmul(v, gk);
+      dlt[i] = v;
+    }
+  }
+  __syncthreads();
+}
+
 // One tile of a register-layout pass: ops[pd->op_begin .. + pd->nops) are
 // ROps; TileOp primitives (Q_LDS, Q_DIAG) index `tops`.  The op list has the
 // form
@@ -1095,7 +1098,7 @@ __device__ __forceinline__ void reg_tile(cx<T> *__restrict__ sv, const PassDesc 
                                          const TileOp *__restrict__ tops,
                                          const cx<double> *__restrict__ pool,
                                          const ProbeSink &sink, uint64_t tile_id, cx<T> *lds,
-                                         cx<T> *mat, uint32_t *doff, double *red, cx<T> *gsh) {
+                                         cx<T> *mat, uint32_t *doff, double *red, cx<double> *dlt) {
   constexpr int TB = B - R;
   constexpr int NT = 1 << TB;
   constexpr int NR = 1 << R;
@@ -1118,10 +1121,16 @@ __device__ __forceinline__ void reg_tile(cx<T> *__restrict__ sv, const PassDesc 
     LayoutState<NR> st = make_state<R, TB>(e.u.lay, tid);
     // The lane records are read before the tile, so that their wait does not
     // wait for the tile loads; the table entries after, overlapping them.
-    const bool has_df = e.a != 0;
+    const uint32_t nslot = e.a;
     const cx<double> *dlane = pool + e.b;
     uint2 drec = make_uint2(0u, 0u);
-    if (has_df) drec = reinterpret_cast<const uint2 *>(dlane)[tid & 63u];
+    uint32_t sb[256 / NT] = {};
+    if (nslot) {
+      drec = reinterpret_cast<const uint2 *>(dlane)[tid & 63u];
+      const uint32_t *stage_back = reinterpret_cast<const uint32_t *>(dlane + 32);
+#pragma unroll
+      for (int j = 0; j < 256 / NT; ++j) sb[j] = stage_back[j * NT + tid];
+    }
     Amps<T, NR> a;
     if (e.kind == Q_LOAD) {
       const RegOff<NR> ro = reg_offset<R>(e.u.lay);
@@ -1132,16 +1141,7 @@ __device__ __forceinline__ void reg_tile(cx<T> *__restrict__ sv, const PassDesc 
 #pragma unroll
       for (int j = 0; j < NR; ++j) a.v[j] = lds[st.tl | st.ri.at(j)];
     }
-    // held in the amplitude precision (complex64: two registers or 8 bytes)
-    cx<T> gdf = mk<T>(1, 0);
-    if (has_df) {
-      const cx<T> g = to_t<T>(slot_factor(drec, dlane, base));
-      if constexpr (diag_in_lds<T, FULL>()) {
-        if ((tid & 7u) == 0) gsh[(tid >> 6) * kDiagSlots + ((tid & 63u) >> 3)] = g;
-      } else {
-        gdf = g;
-      }
-    }
+    if (nslot) stage_diag_tables<NT>(drec, dlane, nslot, base, tid, sb, dlt);
     // Register ops alternate between banks a and b.  The head of the op
     // after the current one is loaded while the current op runs.
     const uint32_t cnt = e.extra;
@@ -1150,9 +1150,9 @@ __device__ __forceinline__ void reg_tile(cx<T> *__restrict__ sv, const PassDesc 
     for (uint32_t i = 0; i < cnt; i += 2) {
       const OpHead<T, !FULL> h1 = load_head<!FULL>(opp[1]);
       const Amps<T, NR> b =
-          rw_op<T, B, R, FULL>(opp[0], h0, a, st, lds, red, base, tid, tops, pool, sink, gdf, gsh);
+          rw_op<T, B, R, FULL>(opp[0], h0, a, st, lds, red, base, tid, tops, pool, sink, dlt);
       h0 = load_head<!FULL>(opp[2]);
-      a = rw_op<T, B, R, FULL>(opp[1], h1, b, st, lds, red, base, tid, tops, pool, sink, gdf, gsh);
+      a = rw_op<T, B, R, FULL>(opp[1], h1, b, st, lds, red, base, tid, tops, pool, sink, dlt);
       opp += 2;
     }
     o += cnt;
@@ -1183,6 +1183,9 @@ __device__ __forceinline__ void reg_tile(cx<T> *__restrict__ sv, const PassDesc 
   }
 }
 
+// Dynamic LDS of the pass kernels (the fused-diagonal lane tables).
+extern __shared__ __align__(16) unsigned char aerhip_dyn_lds[];
+
 // Waves per SIMD that the tile's LDS allows (160 KiB per CU, four SIMDs;
 // 1 KiB per workgroup for the small arrays beside the tile); register
 // allocation is held to that occupancy.  WIDE: the lean variant, which uses
@@ -1211,11 +1214,11 @@ __attribute__((amdgpu_waves_per_eu(reg_pass_waves<T, B, R, WIDE>()))) void k_reg
   __shared__ cx<T> mat[16];
   __shared__ uint32_t doff[32];
   __shared__ double red[(1 << (B - R)) / 64];
-  __shared__ cx<T> gsh[(1 << (B - R)) / 64 * kDiagSlots];
   const StateTile st = state_tile(tile_log);
   const ProbeSink sink{probes + st.b * probes_per_state * st.ntiles, st.t, st.ntiles, false};
+  // kDiagLdsBytes of dynamic LDS when the pass has fused-diagonal slots
   reg_tile<T, B, R, 16, FULL>(sv + st.b * state_stride, pd + st.b * pd_stride, rops, tops, pool,
-                              sink, st.t, lds, mat, doff, red, gsh);
+                              sink, st.t, lds, mat, doff, red, reinterpret_cast<cx<double> *>(aerhip_dyn_lds));
 }
 
 // A layer of SWAPs on disjoint qubit pairs permutes the index bits by an
