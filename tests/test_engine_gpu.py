@@ -272,6 +272,8 @@ def test_register_layouts_match_oracle(precision, reg_bits):
         m = D.random_unitary(4, rng)
         e.apply_matrix([a + 1, a], m); o.apply_matrix([a + 1, a], m)
     e.apply_mcswap([3, 12]); o.apply_mcswap([3, 12])
+    for q in range(n // 2):  # a reversal: one bit-permutation pass
+        e.apply_mcswap([q, n - 1 - q]); o.apply_mcswap([q, n - 1 - q])
     # controlled gates with controls inside and outside the tile, Y, and a
     # 2-qubit diagonal
     for ctrl, tgt in ((9, 1), (15, 2), (0, 14), (13, 7)):
@@ -673,3 +675,22 @@ def test_swap_layers_as_permutation_passes(precision, n):
                 e.apply_mcswap([c, a, b])
                 o.apply_mcswap([c, a, b])
     close(e, o, precision)
+
+
+@pytest.mark.parametrize("tile_bits,low_bits", [(6, 2), (8, 3), (9, 1), (11, 3)])
+def test_swap_layers_under_tile_settings(tile_bits, low_bits):
+    """Bit-permutation passes under other tile sizes and low-qubit counts (the
+    permutation tile holds at most tile_bits - 1 qubits), between gates."""
+    n = 14
+    print(f"case: swap layers tile={tile_bits} low={low_bits}", flush=True)
+    e, o, rng = pair(n, 60 + tile_bits)
+    e.set_tile_bits(tile_bits, low_bits)
+    for layer in range(3):
+        for q in range(n):
+            m = D.u3(*rng.uniform(0, 2 * np.pi, 3))
+            e.apply_mcu([q], m)
+            o.apply_mcu([q], m)
+        for a, b in swap_layer(n, 7 - layer, rng):
+            e.apply_mcswap([a, b])
+            o.apply_mcswap([a, b])
+    close(e, o)
