@@ -181,6 +181,30 @@ __device__ __forceinline__ cx<T> mv2(cx<T> m0, cx<T> a0, cx<T> m1, cx<T> a1) {
   return mk<T>(x, y);
 }
 
+// m0*a0 + m1*a1 + m2*a2 + m3*a3: one multiply and seven fused multiply-adds
+// per component.
+template <typename T>
+__device__ __forceinline__ cx<T> mv4(cx<T> m0, cx<T> a0, cx<T> m1, cx<T> a1, cx<T> m2, cx<T> a2, cx<T> m3,
+                                     cx<T> a3) {
+  T x = m3.x * a3.x;
+  x = fma_t(-m3.y, a3.y, x);
+  x = fma_t(m2.x, a2.x, x);
+  x = fma_t(-m2.y, a2.y, x);
+  x = fma_t(m1.x, a1.x, x);
+  x = fma_t(-m1.y, a1.y, x);
+  x = fma_t(m0.x, a0.x, x);
+  x = fma_t(-m0.y, a0.y, x);
+  T y = m3.x * a3.y;
+  y = fma_t(m3.y, a3.x, y);
+  y = fma_t(m2.x, a2.y, y);
+  y = fma_t(m2.y, a2.x, y);
+  y = fma_t(m1.x, a1.y, y);
+  y = fma_t(m1.y, a1.x, y);
+  y = fma_t(m0.x, a0.y, y);
+  y = fma_t(m0.y, a0.x, y);
+  return mk<T>(x, y);
+}
+
 template <typename T>
 __device__ __forceinline__ cx<T> csel(bool c, cx<T> a, cx<T> b) {
   return mk<T>(c ? a.x : b.x, c ? a.y : b.y);
@@ -493,6 +517,24 @@ __device__ __forceinline__ Amps<T, NR> q_u2(const Amps<T, NR> &a, uint32_t tlc,
                                             const cx<double> *__restrict__ mp) {
   constexpr int lo = b0 < b1 ? b0 : b1, hi = b0 < b1 ? b1 : b0;
   Amps<T, NR> o;
+  if (cl == 0) {
+    // no local control: the whole tile or nothing (a uniform branch), and
+    // no per-output selects
+    if (cvl != 0) return pass_through(a);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const cx<T> c0 = to_t<T>(mp[r]), c1 = to_t<T>(mp[r + 4]);
+      const cx<T> c2 = to_t<T>(mp[r + 8]), c3 = to_t<T>(mp[r + 12]);
+#pragma unroll
+      for (int p = 0; p < NR / 4; ++p) {
+        const int j = pair_lo<hi>(pair_lo<lo>(p));
+        const int i1 = j | (1 << b0), i2 = j | (1 << b1), i3 = j | (1 << b0) | (1 << b1);
+        const int dst = r == 0 ? j : r == 1 ? i1 : r == 2 ? i2 : i3;
+        o.v[dst] = mv4(c0, a.v[j], c1, a.v[i1], c2, a.v[i2], c3, a.v[i3]);
+      }
+    }
+    return o;
+  }
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const cx<T> c0 = to_t<T>(mp[r]), c1 = to_t<T>(mp[r + 4]);
@@ -502,10 +544,8 @@ __device__ __forceinline__ Amps<T, NR> q_u2(const Amps<T, NR> &a, uint32_t tlc,
       const int j = pair_lo<hi>(pair_lo<lo>(p));
       const int i1 = j | (1 << b0), i2 = j | (1 << b1), i3 = j | (1 << b0) | (1 << b1);
       const int dst = r == 0 ? j : r == 1 ? i1 : r == 2 ? i2 : i3;
-      const cx<T> s01 = mv2(c0, a.v[j], c1, a.v[i1]);
-      const cx<T> s23 = mv2(c2, a.v[i2], c3, a.v[i3]);
       const bool on = (tlc | (ri.at(j) & cl)) == cvl;
-      o.v[dst] = csel(on, mk<T>(s01.x + s23.x, s01.y + s23.y), a.v[dst]);
+      o.v[dst] = csel(on, mv4(c0, a.v[j], c1, a.v[i1], c2, a.v[i2], c3, a.v[i3]), a.v[dst]);
     }
   }
   return o;
