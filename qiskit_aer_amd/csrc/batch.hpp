@@ -34,11 +34,15 @@ __attribute__((amdgpu_waves_per_eu(reg_pass_waves<T, B, R>()))) void k_batch(cx<
                                                     double *__restrict__ probes) {
   static_assert((1 << (B - R)) == kThreads, "register passes and LDS passes share the workgroup");
   __shared__ cx<T> lds[1 << B];
-  __shared__ uint64_t hioff[256];
+  // the LDS-pass offset table and the register passes' staged tables never
+  // live at the same time: one array holds either
+  __shared__ __align__(16) unsigned char aux_lds[kDiagLdsBytes > 256 * sizeof(uint64_t) ? kDiagLdsBytes
+                                                                                         : 256 * sizeof(uint64_t)];
+  uint64_t *hioff = reinterpret_cast<uint64_t *>(aux_lds);
+  cx<double> *dlt = reinterpret_cast<cx<double> *>(aux_lds);
   __shared__ cx<T> mat[16];
   __shared__ uint32_t doff[32];
   __shared__ double red[kThreads / 64];
-  __shared__ cx<double> dlt[kDiagSlots * 32];
   const BatchCircuit c = circ[blockIdx.x];
   cx<T> *sv = states + c.sv_off;
   const uint64_t size = 1ull << c.num_qubits;

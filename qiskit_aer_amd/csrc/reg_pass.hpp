@@ -1106,8 +1106,8 @@ __device__ __forceinline__ cx<double> slot_factor(uint2 rec, const cx<double> *d
 
 // Stages the lane tables of the phase's fused-diagonal slots in LDS:
 // dlt[32k + e] = entry e of slot k's lane tables, entries 0-15 times the
-// slot's fixed-qubit factor (which lanes 8k..8k+7 of every wave hold), then
-// waits for the workgroup.  Staging thread i reads the entry at distance
+// slot's fixed-qubit factor (which lanes 8k..8k+7 of every wave hold); the
+// caller waits for the workgroup.  Staging thread i reads the entry at distance
 // sb[i / NT] back from the records (read before the tile loads).
 template <int NT>
 __device__ __forceinline__ void stage_diag_tables(uint2 drec, const cx<double> *dlane, uint32_t nslot,
@@ -1124,7 +1124,6 @@ __device__ __forceinline__ void stage_diag_tables(uint2 drec, const cx<double> *
       dlt[i] = v;
     }
   }
-  __syncthreads();
 }
 
 // One tile of a register-layout pass: ops[pd->op_begin .. + pd->nops) are
@@ -1181,7 +1180,13 @@ __device__ __forceinline__ void reg_tile(cx<T> *__restrict__ sv, const PassDesc 
 #pragma unroll
       for (int j = 0; j < NR; ++j) a.v[j] = lds[st.tl | st.ri.at(j)];
     }
-    if (nslot) stage_diag_tables<NT>(drec, dlane, nslot, base, tid, sb, dlt);
+    if (nslot) {
+      // A workgroup that loops over tiles may still read the previous tile's
+      // tables: wait before overwriting them (a FILL phase waited already).
+      if (e.kind == Q_LOAD) __syncthreads();
+      stage_diag_tables<NT>(drec, dlane, nslot, base, tid, sb, dlt);
+      __syncthreads();
+    }
     // Register ops alternate between banks a and b.  The head of the op
     // after the current one is loaded while the current op runs.
     const uint32_t cnt = e.extra;

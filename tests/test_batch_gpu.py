@@ -77,3 +77,24 @@ def test_batch_with_swap_layers(precision):
     want = np.array([oracle_energy(c, h) for c, h in zip(cs, hs)])
     tol = 1e-10 if precision == "double" else 1e-4
     assert np.max(np.abs(e_batch - want)) < tol
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_batch_with_staged_tables(precision):
+    """QFT ladders (fused diagonals with lane tables staged in LDS) and SU(4)
+    layers in circuits of many tiles, where one workgroup walks all tiles of
+    its circuit and restages the tables per tile."""
+    sizes = [12, 14, 16, 13]
+    print(f"case: batch staged tables {precision} sizes={sizes}", flush=True)
+    cs, hs = [], []
+    for i, n in enumerate(sizes):
+        c = circuits.qft(n, swaps=False)
+        c.ops += circuits.su4_brickwork(n, 3, 50 + i).ops
+        cs.append(c)
+        hs.append(circuits.random_pauli_sum(n, 8, 70 + i))
+    e_batch, _ = batch_pauli_expval(cs, hs, precision, "batch")
+    e_seq, _ = batch_pauli_expval(cs, hs, precision, "sequential")
+    want = np.array([oracle_energy(c, h) for c, h in zip(cs, hs)])
+    tol = 1e-10 if precision == "double" else 1e-4
+    assert np.max(np.abs(e_batch - want)) < tol, (e_batch, want)
+    assert np.max(np.abs(e_seq - want)) < tol, (e_seq, want)
