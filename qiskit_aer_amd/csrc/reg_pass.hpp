@@ -1274,57 +1274,55 @@ __attribute__((amdgpu_waves_per_eu(reg_pass_waves<T, B, R, WIDE>()))) void k_reg
 // so the pass is in place without races.  Both tiles go through LDS: 2^SB
 // amplitudes each.
 template <typename T, int SB, int NT>
-__device__ __forceinline__ void qperm_tile(cx<T> *__restrict__ sv, const PassDesc *__restrict__ pdp,
-                                           const TileOp *__restrict__ opp, uint64_t t, cx<T> *lds) {
-  const uint32_t tid = threadIdx.x;
+struct QPerm {
+  static constexpr int NE = (1 << SB) >= NT ? (1 << SB) / NT : 1;  // elements per thread and tile
+  uint32_t tid, ns, npairs, N;
+  uint64_t cg;
+  uint32_t xl, zl;
+  uint32_t sp[SB], px[16], py[16];
+  uint64_t et;     // expand() of the thread's part of the local index
+  uint32_t wt, rt;  // its LDS slots for the writes and the permuted reads
+
   // The descriptors, read once into scalar registers (compile-time loop
   // bounds keep them there).
-  const uint32_t ns = pdp->nb, npairs = opp->k;
-  const uint64_t cg = opp->cg;
-  const uint32_t xl = opp->xl, zl = opp->zl;
-  uint32_t sp[SB], px[16], py[16];
+  __device__ __forceinline__ void init(const PassDesc *__restrict__ pdp, const TileOp *__restrict__ opp) {
+    tid = threadIdx.x;
+    ns = pdp->nb;
+    npairs = opp->k;
+    cg = opp->cg;
+    xl = opp->xl;
+    zl = opp->zl;
 #pragma unroll
-  for (int i = 0; i < SB; ++i) sp[i] = pdp->spos[i];
+    for (int i = 0; i < SB; ++i) sp[i] = pdp->spos[i];
 #pragma unroll
-  for (int j = 0; j < 16; ++j) {
-    px[j] = opp->pos[j];
-    py[j] = opp->gpos[j];
-  }
-  // base of tile t: t with zeros inserted at the tile qubits (ascending)
-  uint64_t b = t;
-#pragma unroll
-  for (int i = 0; i < SB; ++i)
-    if (static_cast<uint32_t>(i) < ns) {
-      const uint32_t q = sp[i];
-      b = ((b >> q) << (q + 1)) | (b & ((1ull << q) - 1ull));
+    for (int j = 0; j < 16; ++j) {
+      px[j] = opp->pos[j];
+      py[j] = opp->gpos[j];
     }
-  uint64_t pb = b;
-#pragma unroll
-  for (int j = 0; j < 16; ++j)
-    if (static_cast<uint32_t>(j) < npairs && (((b >> px[j]) ^ (b >> py[j])) & 1ull))
-      pb ^= (1ull << px[j]) | (1ull << py[j]);
-  b = uniform64(b);
-  pb = uniform64(pb);
-  if (pb < b) return;  // the workgroup of tile pb exchanges the pair (uniform)
-  const uint32_t N = 1u << ns;
+    N = 1u << ns;
+    const uint32_t tk = tid & (N - 1u);
+    et = expand(tk);
+    wt = swz(tk);
+    rt = swz(lperm(tk));
+  }
   // The maps below are linear in the local index: the thread's part once,
-  // the part of each step (k = tid + k0, k0 a multiple of NT) in the loop.
-  auto expand = [&](uint32_t k) {
+  // the part of each step (k = tid + k0, k0 a multiple of NT) per step.
+  __device__ __forceinline__ uint64_t expand(uint32_t k) const {
     uint64_t e = 0;
 #pragma unroll
     for (int i = 0; i < SB; ++i)
       if (static_cast<uint32_t>(i) < ns) e |= static_cast<uint64_t>((k >> i) & 1u) << sp[i];
     return e;
-  };
-  auto lperm = [&](uint32_t k) {
+  }
+  __device__ __forceinline__ uint32_t lperm(uint32_t k) const {
     uint32_t m = 0;
 #pragma unroll
     for (int i = 0; i < SB; ++i)
       if (static_cast<uint32_t>(i) < ns) m |= ((k >> i) & 1u) << ((cg >> (4 * i)) & 15u);
     return m;
-  };
+  }
   // LDS slot of local index k: k ^ g(k) (bank-conflict-free permuted reads)
-  auto swz = [&](uint32_t k) {
+  __device__ __forceinline__ uint32_t swz(uint32_t k) const {
     uint32_t g = 0;
 #pragma unroll
     for (int j = 0; j < 5; ++j) {
@@ -1332,42 +1330,126 @@ __device__ __forceinline__ void qperm_tile(cx<T> *__restrict__ sv, const PassDes
       if (p < 32) g |= ((k >> p) & 1u) << j;
     }
     return k ^ g;
-  };
-  const uint32_t tk = tid & (N - 1u);
-  const uint64_t et = expand(tk);
-  const uint32_t wt = swz(tk), rt = swz(lperm(tk));
-  const bool pair = pb != b;
-  for (uint32_t k0 = 0; k0 < N; k0 += NT) {
-    if (tid >= N) break;
-    const uint64_t e = et | expand(k0);
-    const uint32_t w = wt ^ swz(k0);
-    lds[w] = ld_stream(sv + b + e);
-    if (pair) lds[N + w] = ld_stream(sv + pb + e);
   }
+  // Base of tile t (t with zeros inserted at the tile qubits) and of its
+  // partner tile (the pairs outside the tile exchanged); both uniform.
+  __device__ __forceinline__ void bases(uint64_t t, uint64_t &b, uint64_t &pb) const {
+    b = t;
+#pragma unroll
+    for (int i = 0; i < SB; ++i)
+      if (static_cast<uint32_t>(i) < ns) {
+        const uint32_t q = sp[i];
+        b = ((b >> q) << (q + 1)) | (b & ((1ull << q) - 1ull));
+      }
+    pb = b;
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+      if (static_cast<uint32_t>(j) < npairs && (((b >> px[j]) ^ (b >> py[j])) & 1ull))
+        pb ^= (1ull << px[j]) | (1ull << py[j]);
+    b = uniform64(b);
+    pb = uniform64(pb);
+  }
+  // This thread's amplitudes of tiles b and pb into registers.
+  __device__ __forceinline__ void load(const cx<T> *sv, uint64_t b, uint64_t pb, cx<T> (&va)[NE],
+                                       cx<T> (&vb)[NE]) const {
+#pragma unroll
+    for (int s = 0; s < NE; ++s) {
+      const uint32_t k0 = static_cast<uint32_t>(s) * NT;
+      if (k0 < N && tid < N) {
+        const uint64_t e = et | expand(k0);
+        va[s] = ld_stream(sv + b + e);
+        if (pb != b) vb[s] = ld_stream(sv + pb + e);
+      }
+    }
+  }
+  __device__ __forceinline__ void to_lds(cx<T> *lds, bool pair, const cx<T> (&va)[NE], const cx<T> (&vb)[NE]) const {
+#pragma unroll
+    for (int s = 0; s < NE; ++s) {
+      const uint32_t k0 = static_cast<uint32_t>(s) * NT;
+      if (k0 < N && tid < N) {
+        const uint32_t w = wt ^ swz(k0);
+        lds[w] = va[s];
+        if (pair) lds[N + w] = vb[s];
+      }
+    }
+  }
+  // Output tile pb at local k = tile b at the permuted local index, and the
+  // other way round.
+  __device__ __forceinline__ void store(cx<T> *sv, uint64_t b, uint64_t pb, const cx<T> *lds) const {
+#pragma unroll
+    for (int s = 0; s < NE; ++s) {
+      const uint32_t k0 = static_cast<uint32_t>(s) * NT;
+      if (k0 < N && tid < N) {
+        const uint64_t e = et | expand(k0);
+        const uint32_t r = rt ^ swz(lperm(k0));
+        st_stream(sv + pb + e, lds[r]);
+        if (pb != b) st_stream(sv + b + e, lds[N + r]);
+      }
+    }
+  }
+};
+
+// One tile (the batched-circuit kernel's tile loop): the workgroup of tile t
+// exchanges it with its partner, or skips it when the partner's base is the
+// smaller one (uniform).
+template <typename T, int SB, int NT>
+__device__ __forceinline__ void qperm_tile(cx<T> *__restrict__ sv, const PassDesc *__restrict__ pdp,
+                                           const TileOp *__restrict__ opp, uint64_t t, cx<T> *lds) {
+  QPerm<T, SB, NT> q;
+  q.init(pdp, opp);
+  uint64_t b, pb;
+  q.bases(t, b, pb);
+  if (pb < b) return;
+  cx<T> va[QPerm<T, SB, NT>::NE], vb[QPerm<T, SB, NT>::NE];
+  q.load(sv, b, pb, va, vb);
+  q.to_lds(lds, pb != b, va, vb);
   __syncthreads();
-  for (uint32_t k0 = 0; k0 < N; k0 += NT) {
-    if (tid >= N) break;
-    const uint64_t e = et | expand(k0);
-    const uint32_t r = rt ^ swz(lperm(k0));
-    st_stream(sv + pb + e, lds[r]);
-    if (pair) st_stream(sv + b + e, lds[N + r]);
-  }
-  __syncthreads();  // the LDS tile is free again
+  q.store(sv, b, pb, lds);
+  __syncthreads();  // the LDS tiles are free again
 }
 
-// 2^per_wg_log consecutive tiles per workgroup (about half of them are the
-// larger base of their pair and skipped, so one tile per workgroup would
-// leave half the workgroups idle after their set-up).
+// Up to four consecutive tiles per workgroup (about half of all tiles are the
+// larger base of their pair and skipped), software-pipelined: the loads of
+// the next exchanged tile pair are issued before the stores of the current
+// one.
 template <typename T, int SB>
 __global__ __launch_bounds__(kThreads) void k_qperm(cx<T> *__restrict__ sv, const PassDesc *__restrict__ pd,
                                                     const TileOp *__restrict__ ops, uint64_t state_stride,
                                                     uint32_t pd_stride, uint32_t tile_log,
                                                     uint32_t per_wg_log) {
+  using Q = QPerm<T, SB, kThreads>;
   __shared__ cx<T> lds[2 << SB];
   const StateTile st = state_tile(tile_log);
   const PassDesc *d = pd + st.b * pd_stride;
-  for (uint32_t i = 0; i < (1u << per_wg_log); ++i)
-    qperm_tile<T, SB, kThreads>(sv + st.b * state_stride, d, ops + d->op_begin, (st.t << per_wg_log) + i, lds);
+  cx<T> *s = sv + st.b * state_stride;
+  Q q;
+  q.init(d, ops + d->op_begin);
+  const uint32_t count = 1u << per_wg_log;  // at most 4
+  uint64_t b[4], pb[4];
+  bool act[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    b[i] = pb[i] = 0;
+    act[i] = false;
+    if (static_cast<uint32_t>(i) < count) {
+      q.bases((st.t << per_wg_log) + static_cast<uint32_t>(i), b[i], pb[i]);
+      act[i] = pb[i] >= b[i];
+    }
+  }
+  cx<T> va[Q::NE], vb[Q::NE];
+  if (act[0]) q.load(s, b[0], pb[0], va, vb);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    if (act[i]) {
+      q.to_lds(lds, pb[i] != b[i], va, vb);
+      __syncthreads();
+    }
+    if (i + 1 < 4 && act[i + 1]) q.load(s, b[i + 1], pb[i + 1], va, vb);  // the registers are free again
+    if (act[i]) {
+      q.store(s, b[i], pb[i], lds);
+      __syncthreads();  // the LDS tiles are free again
+    }
+  }
 }
 
 }  // namespace aerhip
