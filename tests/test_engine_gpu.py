@@ -567,15 +567,20 @@ def test_norm_of_matrix_and_diagonal_and_amplitude_access(precision):
 
 @pytest.mark.parametrize("precision", ["double", "single"])
 @pytest.mark.parametrize("max_qubit", [64, 3])
-def test_fused_diagonal_runs_match_oracle(precision, max_qubit):
+@pytest.mark.parametrize("more_registers", [False, True])
+def test_fused_diagonal_runs_match_oracle(precision, max_qubit, more_registers):
     """Runs of phase gates (cp / cz / p on pairs and single qubits), RZ, S, T
     and single-qubit projectors between 2x2 gates and saved values, with
     qubits on register bits, thread bits and qubits fixed by the tile: the
     engine fuses them into table-driven diagonal ops (or, with max_qubit 3,
-    into short ones) and must give the oracle's state and saved values."""
+    into short ones) and must give the oracle's state and saved values; also
+    with one more register bit (128-thread workgroups staging the tables)."""
     n = 17
-    print(f"case: fused diagonals {precision} max_qubit={max_qubit}", flush=True)
+    print(f"case: fused diagonals {precision} max_qubit={max_qubit} more_registers={more_registers}",
+          flush=True)
     e, o, rng = pair(n, 18, precision)
+    if more_registers:
+        e.set_register_bits(4 if precision == "double" else 5)
     e.set_fusion(True, max_qubit, 0)
     want = []
     for rnd in range(6):
