@@ -38,9 +38,12 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_engine(n: int, precision: str, circ, steps: int, warmup: int, init=None) -> dict:
+def run_engine(n: int, precision: str, circ, steps: int, warmup: int, init=None,
+               fusion_width: int = 0) -> dict:
     from qiskit_aer_amd import QubitVector
     qv = QubitVector(n, precision)
+    if fusion_width:
+        qv.set_block_fusion(fusion_width, cost_check=True)
     if init is None:
         qv.initialize()
     else:
@@ -149,7 +152,7 @@ def run_mixed_precision(n: int, layers: int, seed: int) -> dict:
     width = {}
     out["autotune"] = {}
     for prec in ("double", "single"):
-        best, times = tune(proxy, prec, widths=(0, 4, 5))
+        best, times = tune(proxy, prec, widths=(0, 2, 4, 5))
         width[prec] = best
         out["autotune"][prec] = {"proxy_qubits": proxy.num_qubits, "best_width": best,
                                  "ms_by_width": times}
@@ -294,8 +297,11 @@ def main() -> int:
                     help="run only this leg (for profiling it) and print its figures")
     ap.add_argument("--precision", choices=("double", "single"), default="double",
                     help="precision of the su4 leg")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the block-fusion autotuner of the brickwork legs (use --fusion-width)")
     ap.add_argument("--fusion-width", type=int, default=0,
-                    help="block-fusion width of the su4 leg (0: off, 2..5)")
+                    help="block-fusion width of the su4 and c64 legs, and of the brickwork with "
+                         "--no-tune (0: off, 2..5)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -332,11 +338,25 @@ def main() -> int:
         return 0
     circ = circuits.brickwork_u3_cx(n, args.layers, seed, measure=False)
     if args.leg == "c64":
-        res = run_engine(n, "single", circ, args.steps, args.warmup)
+        res = run_engine(n, "single", circ, args.steps, args.warmup, fusion_width=args.fusion_width)
         print(json.dumps({"leg": f"brickwork_{n}q_depth{args.layers}_c64", **res}), flush=True)
         return 0
     log(f"bench: brickwork {n}q depth {args.layers} c128, {circ.count_gates()} gates")
-    c128 = run_engine(n, "double", circ, args.steps, args.warmup)
+    # Block-fusion width per precision (0 = off, 2..5 qubits per fused gate,
+    # the configured fusion limit being 5), picked by the autotuner on the
+    # circuit itself before the timed runs.
+    from qiskit_aer_amd.autotune import tune
+    widths = {}
+    for prec in ("double",) if args.no_extra else ("double", "single"):
+        if args.no_tune:
+            widths[prec] = {"best_width": args.fusion_width, "ms_by_width": {}}
+            continue
+        best, times = tune(circ, prec, widths=(0, 2, 3, 4, 5))
+        widths[prec] = {"best_width": best, "ms_by_width": times}
+        log(f"bench: fusion autotune {prec}: width {best}, " +
+            ", ".join(f"{w}: {t:.0f} ms" for w, t in times.items()))
+    c128 = run_engine(n, "double", circ, args.steps, args.warmup,
+                      fusion_width=widths["double"]["best_width"])
     # FP64 work of the step: each U3 costs 8 fused multiply-adds per amplitude
     # (a complex 2x2 on each pair), CX only moves amplitudes.
     u3_per_step = n * args.layers
@@ -344,13 +364,19 @@ def main() -> int:
     log(f"bench: c128 {c128['gates_per_s']:.1f} gates/s, {c128['hbm_gbs']:.0f} GB/s, "
         f"kernel {c128['kernel_gbs']:.0f} GB/s, {c128['passes_per_step']:.0f} passes, "
         f"{c128['gathers_per_step']:.0f} register layouts per step")
-    extra = {}
+    extra = {"fusion_autotune": widths}
     if not args.no_extra:
-        c64 = run_engine(n, "single", circ, args.steps, args.warmup)
-        log(f"bench: c64 {c64['gates_per_s']:.1f} gates/s, {c64['hbm_gbs']:.0f} GB/s")
+        w64 = widths["single"]["best_width"]
+        c64 = run_engine(n, "single", circ, args.steps, args.warmup, fusion_width=w64)
+        log(f"bench: c64 {c64['gates_per_s']:.1f} gates/s, {c64['hbm_gbs']:.0f} GB/s (fusion width {w64})")
         extra["c64"] = {k: c64[k] for k in ("gates_per_s", "ms_per_step", "hbm_gbs", "kernel_gbs",
                                              "passes_per_step")}
+        extra["c64"]["fusion_width"] = w64
         extra["c64_over_c128"] = c64["gates_per_s"] / c128["gates_per_s"]
+        if w64:
+            u64 = run_engine(n, "single", circ, args.steps, args.warmup)
+            extra["c64_unfused"] = {k: u64[k] for k in ("gates_per_s", "ms_per_step", "kernel_gbs",
+                                                         "passes_per_step")}
         q = circuits.qft(n)
 
         def gaussian(qv):
@@ -414,7 +440,7 @@ def main() -> int:
         "data": "synthetic (seeded random U3 angles)",
         "config": {"workload": f"brickwork_u3_cx_{n}q_depth{args.layers}", "num_qubits": n,
                    "layers": args.layers, "gates": circ.count_gates(), "precision": "c128",
-                   "parallelism": "single GPU"},
+                   "fusion_width": widths["double"]["best_width"], "parallelism": "single GPU"},
         "hbm_gbs": c128["hbm_gbs"],
         "passes_per_step": c128["passes_per_step"],
         "roofline": {"bound": "hbm", "achieved": c128["kernel_gbs"], "peak": HBM_PEAK_GBS,

@@ -15,7 +15,7 @@ from typing import Iterable, Optional
 from .circuit import Circuit
 
 
-def tune(circuit: Circuit, precision: str = "double", widths: Iterable[int] = (0, 3, 4, 5),
+def tune(circuit: Circuit, precision: str = "double", widths: Iterable[int] = (0, 2, 3, 4, 5),
          reps: int = 1, qv=None, cost_check: bool = True) -> tuple[int, dict]:
     """Returns (best width, {width: device ms per run}).  `qv` (optional) is a
     QubitVector of the circuit's size to time on (it is re-initialised); the
