@@ -29,9 +29,10 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X published FP64 peak (vector and matrix)
 
 # HBM bytes per tile-pass launch of the default workload (30 qubits, c128),
 # from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (profiles/reg_pass_30q_brickwork_c128.md):
-# 2.0 x 8 388 809 KiB fetched (FETCH_SIZE counts half the bytes, calibrated on a
-# kernel that reads the 16 GiB state once) + 16 777 668 KiB written.
-MEASURED_TRAFFIC_30Q_C128 = (2 * 8388809 + 16777668) * 1024
+# 2.0 x 8 389 015 KiB fetched (FETCH_SIZE counts half the bytes, calibrated on a
+# kernel that reads the 16 GiB state once) + 17 450 510 KiB written, for the
+# block-fused pass at width 2 (the unfused pass writes 16 777 668 KiB).
+MEASURED_TRAFFIC_30Q_C128 = (2 * 8389015 + 17450510) * 1024
 
 
 def log(msg: str) -> None:
@@ -357,14 +358,21 @@ def main() -> int:
             ", ".join(f"{w}: {t:.0f} ms" for w, t in times.items()))
     c128 = run_engine(n, "double", circ, args.steps, args.warmup,
                       fusion_width=widths["double"]["best_width"])
-    # FP64 work of the step: each U3 costs 8 fused multiply-adds per amplitude
-    # (a complex 2x2 on each pair), CX only moves amplitudes.
+    # FP64 work of the source circuit: each U3 costs 8 fused multiply-adds per
+    # amplitude (a complex 2x2 on each pair), CX only moves amplitudes (fused
+    # 4x4 gates do more arithmetic than their members: this is the lower bound).
     u3_per_step = n * args.layers
     fp64_tflops = u3_per_step * (1 << n) * 16 / (c128["ms_per_step"] / 1e3) / 1e12
     log(f"bench: c128 {c128['gates_per_s']:.1f} gates/s, {c128['hbm_gbs']:.0f} GB/s, "
         f"kernel {c128['kernel_gbs']:.0f} GB/s, {c128['passes_per_step']:.0f} passes, "
         f"{c128['gathers_per_step']:.0f} register layouts per step")
     extra = {"fusion_autotune": widths}
+    if not args.no_extra and widths["double"]["best_width"]:
+        # the same step without block fusion: more passes, each lighter
+        u128 = run_engine(n, "double", circ, args.steps, args.warmup)
+        extra["c128_unfused"] = {k: u128[k] for k in ("gates_per_s", "ms_per_step", "kernel_gbs",
+                                                       "passes_per_step")}
+        extra["c128_unfused"]["roofline_frac"] = u128["kernel_gbs"] / HBM_PEAK_GBS
     if not args.no_extra:
         w64 = widths["single"]["best_width"]
         c64 = run_engine(n, "single", circ, args.steps, args.warmup, fusion_width=w64)
@@ -408,11 +416,13 @@ def main() -> int:
         extra[f"su4_brickwork_{args.mixed_qubits}q"] = mp
         if args.big_qubits:
             big = circuits.brickwork_u3_cx(args.big_qubits, args.layers, seed, measure=False)
-            br = run_engine(args.big_qubits, "double", big, 1, 0)
+            br = run_engine(args.big_qubits, "double", big, 1, 0,
+                            fusion_width=widths["double"]["best_width"])
             log(f"bench: brickwork {args.big_qubits}q c128 {br['gates_per_s']:.1f} gates/s, "
                 f"kernel {br['kernel_gbs']:.0f} GB/s")
             extra[f"brickwork_{args.big_qubits}q_c128"] = {
                 k: br[k] for k in ("gates_per_s", "ms_per_step", "hbm_gbs", "kernel_gbs", "passes_per_step")}
+            extra[f"brickwork_{args.big_qubits}q_c128"]["fusion_width"] = widths["double"]["best_width"]
         bt = run_batched(args.batch_circuits, 64, seed)
         log(f"bench: batch {bt['circuits']} circuits: one launch {bt['batch_c128']['circuits_per_s']:.0f} "
             f"circuits/s (c64 {bt['batch_c64']['circuits_per_s']:.0f}), circuit by circuit "
