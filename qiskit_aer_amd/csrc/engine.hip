@@ -1528,30 +1528,62 @@ std::vector<PassPlan> Engine::plan() const {
     std::vector<uint32_t> remaining;
     for (size_t i = seg_begin; i < seg_end; ++i) remaining.push_back(static_cast<uint32_t>(i));
     while (!remaining.empty()) {
-      uint64_t s = lowmask;
-      uint64_t blocked = 0;
-      bool nonunitary_blocked = false;  // a deferred op rescales amplitudes
+      // One candidate pass: the tile starts as `s0`, then takes the
+      // primitives in program order while their mixed qubits fit.  With
+      // `take` null it only counts, and stops once every qubit the remaining
+      // primitives touch is blocked.
+      uint64_t active = 0;
+      for (uint32_t idx : remaining) active |= queue_[idx].touch;
+      auto scan = [&](uint64_t s0, PassPlan *take, std::vector<uint32_t> *rest) {
+        uint64_t s = s0;
+        uint64_t blocked = 0;
+        bool nonunitary_blocked = false;  // a deferred op rescales amplitudes
+        size_t count = 0;
+        for (size_t r = 0; r < remaining.size(); ++r) {
+          const uint32_t idx = remaining[r];
+          const Prim &p = queue_[idx];
+          bool fits = !(p.touch & blocked) && !(p.kind == K_PROBE && nonunitary_blocked);
+          if (fits) {
+            const uint64_t ns = s | p.tmask;
+            fits = static_cast<uint32_t>(__builtin_popcountll(ns)) <= nb;
+            if (fits) s = ns;
+          }
+          if (fits) {
+            ++count;
+            if (take) take->prims.push_back(idx);
+            continue;
+          }
+          if (rest) rest->push_back(idx);
+          blocked |= p.touch;
+          nonunitary_blocked |= !p.unitary;
+          if (!take && (blocked & active) == active) break;
+        }
+        if (take) take->smask = s;
+        return count;
+      };
+      // Candidates: the tile seeded by nothing but the low qubits, and by
+      // each window of tile_bits - low_bits consecutive qubits.  A window
+      // lets the pass run a whole light cone of a nearest-neighbour circuit
+      // (the 30-qubit U3+CX brickwork takes 29 passes instead of 66); the
+      // pass keeps the candidate that takes the most primitives, the first
+      // on ties.
+      uint64_t seed = lowmask;
+      size_t best = scan(lowmask, nullptr, nullptr);
+      const uint32_t w = nb - low_bits_;
+      if (n_ > nb && w > 0)
+        for (uint32_t a = 0; a + w <= n_; ++a) {
+          const uint64_t s0 = lowmask | (((1ull << w) - 1ull) << a);
+          const size_t c = scan(s0, nullptr, nullptr);
+          if (c > best) {
+            best = c;
+            seed = s0;
+          }
+        }
       PassPlan pp;
       pp.solo = false;
       std::vector<uint32_t> rest;
-      for (uint32_t idx : remaining) {
-        const Prim &p = queue_[idx];
-        if ((p.touch & blocked) || (p.kind == K_PROBE && nonunitary_blocked)) {
-          rest.push_back(idx);
-          blocked |= p.touch;
-          nonunitary_blocked |= !p.unitary;
-          continue;
-        }
-        const uint64_t ns = s | p.tmask;
-        if (static_cast<uint32_t>(__builtin_popcountll(ns)) <= nb) {
-          s = ns;
-          pp.prims.push_back(idx);
-        } else {
-          rest.push_back(idx);
-          blocked |= p.touch;
-          nonunitary_blocked |= !p.unitary;
-        }
-      }
+      scan(seed, &pp, &rest);
+      uint64_t s = pp.smask;
       // Complete the tile with the lowest free qubits (longer contiguous runs).
       for (uint32_t q = 0; q < n_ && static_cast<uint32_t>(__builtin_popcountll(s)) < nb; ++q)
         s |= bit(q);

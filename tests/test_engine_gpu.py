@@ -191,6 +191,25 @@ def test_long_random_sequence_many_passes(precision):
     assert st["passes"] < st["gates"]
 
 
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_brickwork_light_cone_passes(precision):
+    """A nearest-neighbour U3+CX brickwork on 20 qubits: the planner seeds
+    tiles with windows of consecutive qubits, so a pass runs a whole light
+    cone (8 passes for complex128, 7 for complex64; taking primitives in
+    program order from the low qubits alone gives 14 and 10)."""
+    n, layers = 20, 16
+    print(f"case: brickwork light cones n={n} {precision}", flush=True)
+    e, o, rng = pair(n, 11, precision)
+    for layer in range(layers):
+        for q in range(n):
+            m = D.u3(*rng.uniform(0.0, 2.0 * np.pi, 3))
+            e.apply_matrix([q], m); o.apply_matrix([q], m)
+        for a in range(layer % 2, n - 1, 2):
+            e.apply_mcx([a, a + 1]); o.apply_mcx([a, a + 1])
+    close(e, o, precision)
+    assert e.stats()["passes"] <= 8
+
+
 def test_readouts():
     n = 16
     print("case: readouts n=16", flush=True)
