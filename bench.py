@@ -29,10 +29,10 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X published FP64 peak (vector and matrix)
 
 # HBM bytes per tile-pass launch of the default workload (30 qubits, c128),
 # from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (profiles/reg_pass_30q_brickwork_c128.md):
-# 2.0 x 8 389 015 KiB fetched (FETCH_SIZE counts half the bytes, calibrated on a
-# kernel that reads the 16 GiB state once) + 17 450 510 KiB written, for the
-# block-fused pass at width 2 (the unfused pass writes 16 777 668 KiB).
-MEASURED_TRAFFIC_30Q_C128 = (2 * 8389015 + 17450510) * 1024
+# 2.0 x 8 388 973 KiB fetched (FETCH_SIZE counts half the bytes, calibrated on a
+# kernel that reads the 16 GiB state once) + 16 977 358 KiB written, averaged over
+# the passes of the block-fused step at width 2.
+MEASURED_TRAFFIC_30Q_C128 = (2 * 8388973 + 16977358) * 1024
 
 
 def log(msg: str) -> None:
@@ -359,8 +359,9 @@ def main() -> int:
     c128 = run_engine(n, "double", circ, args.steps, args.warmup,
                       fusion_width=widths["double"]["best_width"])
     # FP64 work of the source circuit: each U3 costs 8 fused multiply-adds per
-    # amplitude (a complex 2x2 on each pair), CX only moves amplitudes (fused
-    # 4x4 gates do more arithmetic than their members: this is the lower bound).
+    # amplitude (a complex 2x2 on each pair), CX only moves amplitudes.  Block
+    # fusion executes less: a 4x4 gate costs 16 per amplitude and replaces up to
+    # four U3s with the CX between them.
     u3_per_step = n * args.layers
     fp64_tflops = u3_per_step * (1 << n) * 16 / (c128["ms_per_step"] / 1e3) / 1e12
     log(f"bench: c128 {c128['gates_per_s']:.1f} gates/s, {c128['hbm_gbs']:.0f} GB/s, "
