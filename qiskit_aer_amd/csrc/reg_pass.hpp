@@ -917,10 +917,15 @@ __device__ __forceinline__ Amps<T, 1 << R> rw_op(const ROp<T> &op, const OpHead<
   if (kind == Q_RELAYOUT) {
     // complex64: the tile passes through LDS in a swizzled order, slot x at
     // x ^ ((x >> 4) & 15), so that consecutive lanes reach distinct banks also
-    // when the register bits are the low tile bits.  The swizzle is linear and
-    // the thread and register parts of x are disjoint, so it splits into a
-    // per-lane and a per-register term.
-    auto swz = [](uint32_t x) { return sizeof(T) == 4 ? x ^ ((x >> 4) & 15u) : x; };
+    // when the register bits are the low tile bits; complex128 in the full
+    // variant (the passes with 4x4 gates, which change layout most): slot x at
+    // x ^ ((x >> 3) & 7).  Each swizzle is linear and the thread and register
+    // parts of x are disjoint, so it splits into a per-lane and a per-register
+    // term.
+    auto swz = [](uint32_t x) {
+      if (sizeof(T) == 4) return x ^ ((x >> 4) & 15u);
+      return FULL ? x ^ ((x >> 3) & 7u) : x;
+    };
     __syncthreads();
     const uint32_t tw = swz(st.tl);
 #pragma unroll
