@@ -29,10 +29,10 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X published FP64 peak (vector and matrix)
 
 # HBM bytes per tile-pass launch of the default workload (30 qubits, c128),
 # from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (profiles/reg_pass_30q_brickwork_c128.md):
-# 2.0 x 8 388 973 KiB fetched (FETCH_SIZE counts half the bytes, calibrated on a
-# kernel that reads the 16 GiB state once) + 16 977 358 KiB written, averaged over
+# 2.0 x 8 388 975 KiB fetched (FETCH_SIZE counts half the bytes, calibrated on a
+# kernel that reads the 16 GiB state once) + 16 977 477 KiB written, averaged over
 # the passes of the block-fused step at width 2.
-MEASURED_TRAFFIC_30Q_C128 = (2 * 8388973 + 16977358) * 1024
+MEASURED_TRAFFIC_30Q_C128 = (2 * 8388975 + 16977477) * 1024
 
 
 def log(msg: str) -> None:
